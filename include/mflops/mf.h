@@ -1,0 +1,133 @@
+/*
+ * mflops: MI355X-native compact derivatives and matrix-diagonalisation
+ * Helmholtz/Poisson solver -- C ABI.
+ *
+ * Conventions
+ *   - Fields are FP64, column-major (Fortran order) f(nx,ny,nz), i fastest,
+ *     exactly the memory layout of the reference's type(field)%f.
+ *   - Boundary arrays follow the reference's type(boundary_condition):
+ *     bcx(ny-2,nz-2,2), bcy(nx-2,nz-2,2), bcz(nx-2,ny-2,2).
+ *   - Functions taking a `stream` work on DEVICE pointers and enqueue on that
+ *     hipStream_t (NULL = default stream); they return once enqueued unless
+ *     stated otherwise.  *_host variants take host pointers and block.
+ *   - Every function returns MF_OK (0) or an error code; mf_last_error()
+ *     gives the message of the last failure on the calling thread.
+ *   - bct codes: 1 = Dirichlet, 2 = Neumann (3 = second-derivative variant
+ *     of the reference's extrapolation table).
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   mf_der_create        <- der_coeffs_init      src/class_derivatives/class_derivatives.f90:111
+ *   mf_der_lines         <- der/dder/der_s/dder_s src/class_derivatives/class_derivatives.f90:186-260
+ *   mf_der3d             <- derx..dderz          src/class_field/class_field.f90:629-978
+ *   mf_solver3d_create   <- solver_coeffs_init_3d src/class_solver/class_solver_3d.f90:380
+ *   mf_solver3d_solve    <- solver_3d / solve_3d src/class_field/class_field.f90:1072,
+ *                                               src/class_solver/class_solver_3d.f90:83
+ *   mf_der_coeffs_1d     <- der_coeffs_generic_gobal + pentalu
+ *                                               src/class_derivatives/class_derivatives.f90:330-423
+ *   mf_solver_coeffs_1d  <- solver_coeffs_init   src/class_solver/class_solver_3d.f90:413-466
+ */
+#ifndef MFLOPS_MF_H
+#define MFLOPS_MF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MF_OK = 0,
+  MF_ERR_ARG = 1,      /* invalid argument (sizes, codes, null pointers) */
+  MF_ERR_SETUP = 2,    /* coefficient setup failed (singular Taylor system,
+                          complex spectrum: class_solver_3d.f90:537-540) */
+  MF_ERR_DEVICE = 3,   /* HIP runtime error */
+  MF_ERR_NOCONV = 4    /* mixed-precision refinement did not reach tol */
+};
+
+enum { MF_FP64 = 0, MF_MIXED = 1 };
+
+typedef struct mf_der mf_der;
+typedef struct mf_solver3d mf_solver3d;
+
+typedef struct mf_solve_report {
+  int iterations;    /* low-precision solves done (0 for MF_FP64)            */
+  double residual;   /* final ||r||_2/||b||_2 of the FP64 operator (MIXED)   */
+  double residual0;  /* same after the first low-precision solve (MIXED)     */
+} mf_solve_report;
+
+const char *mf_last_error(void);
+int mf_version(void);
+
+/* ---------------------------------------------------------------------
+ * Host-side coefficient tables (no GPU needed).
+ * --------------------------------------------------------------------- */
+
+/* Packed compact-scheme tables for one line of n points: dl_lu(n,5) is the
+ * pentalu-factored (p,f,c,d,e) with alpha=dl(1,1), beta=dl(n,5); dr(n,7) is
+ * the explicit stencil.  Column-major like the reference's dl(n,5), dr(n,7).
+ * der = 1|2, so = 2|4|6|8. */
+int mf_der_coeffs_1d(const double *grid, int n, int der, int so,
+                     double *dl_lu, double *dr);
+
+/* Solver tables for one direction of nall points (interior n = nall-2):
+ * ddl(n,5), ddr(n,8), eigen(n), vect(n,n), inv_vect(n,n), inv_lhs(n,n),
+ * neum(2,7); all column-major; vect/inv_vect NOT transposed.  Any output
+ * pointer may be NULL. */
+int mf_solver_coeffs_1d(const double *grid, int nall, const int bct[2], int so,
+                        double *ddl, double *ddr, double *eigen, double *vect,
+                        double *inv_vect, double *inv_lhs, double *neum);
+
+/* ---------------------------------------------------------------------
+ * Compact derivatives.
+ * --------------------------------------------------------------------- */
+
+/* solver_mode != 0 also builds the sub-line (_s) tables on grid(2:n-1),
+ * used by the 3-D driver on lines lying in boundary planes. */
+int mf_der_create(const double *grid, int n, int solver_mode, int so, mf_der **out);
+int mf_der_destroy(mf_der *dc);
+
+/* Per-line derivative of nlines lines of length n (n = the handle's size),
+ * line l at x + l*ld.  order 1|2; sub != 0 applies der_s/dder_s (points
+ * 2..n-1 only; out(1), out(n) are left untouched). */
+int mf_der_lines(const mf_der *dc, int order, int sub, const double *x, double *out,
+                 int nlines, long long ld, void *stream);
+
+/* 3-D derivative along axis 0|1|2 (x|y|z) of x(nx,ny,nz) into out.
+ * Solver mode: interior lines use the full scheme, lines in one boundary
+ * plane use the sub-line scheme; points on two boundary planes are set 0. */
+int mf_der3d(const mf_der *dc, int axis, int order, const double *x, double *out,
+             int nx, int ny, int nz, void *stream);
+int mf_der3d_host(const mf_der *dc, int axis, int order, const double *x, double *out,
+                  int nx, int ny, int nz);
+
+/* ---------------------------------------------------------------------
+ * Helmholtz / Poisson solver:  d2u/dx2 + d2u/dy2 + d2u/dz2 + sigma u = g.
+ * --------------------------------------------------------------------- */
+
+/* Full grids gx(nx), gy(ny), gz(nz) including boundary points;
+ * bct = {x1,x2,y1,y2,z1,z2}. */
+int mf_solver3d_create(const double *gx, int nx, const double *gy, int ny,
+                       const double *gz, int nz, const int bct[6], int so,
+                       mf_solver3d **out);
+int mf_solver3d_destroy(mf_solver3d *sc);
+
+/* Solve with g(nx,ny,nz) (only the interior is read; g is not modified),
+ * u(nx,ny,nz) inout (interior and the 6 faces written, the 12 edges left
+ * untouched).  precision MF_FP64: the reference algorithm in FP64.
+ * MF_MIXED: FP32 MFMA transforms with FP64-residual iterative refinement
+ * until ||r||/||b|| <= tol (tol <= 0 selects 1e-12) or max_iter solves.
+ * MIXED synchronises the stream (it reads back residual norms). */
+int mf_solver3d_solve(mf_solver3d *sc, const double *g, double sigma, double *u,
+                      const double *bcx, const double *bcy, const double *bcz,
+                      int precision, double tol, int max_iter,
+                      mf_solve_report *rep, void *stream);
+int mf_solver3d_solve_host(mf_solver3d *sc, const double *g, double sigma, double *u,
+                           const double *bcx, const double *bcy, const double *bcz,
+                           int precision, double tol, int max_iter,
+                           mf_solve_report *rep);
+
+/* Interior sizes of a solver handle (n1 = nx-2, ...). */
+int mf_solver3d_dims(const mf_solver3d *sc, int *n1, int *n2, int *n3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFLOPS_MF_H */

@@ -1,0 +1,41 @@
+// Internal declarations shared by the host setup, the HIP kernels and the
+// C-ABI layer.  Not part of the public interface (include/mflops/mf.h).
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace mf {
+
+// One direction of the diagonalisation solver (the reference's
+// type(solver_coeffs), class_solver_3d.f90:28-44), plus the dense lhs/rhs the
+// iterative-refinement residual needs.
+struct Solver1D {
+  int n = 0;      // interior points
+  int bc[2] = {1, 1};
+  std::vector<double> ddl, ddr;        // (n,5), (n,8) column-major
+  std::vector<double> lhs, rhs;        // dense (n,n)
+  std::vector<double> inv_lhs;         // (n,n)
+  std::vector<double> eigen;           // (n)
+  std::vector<double> vect, inv_vect;  // (n,n), not transposed
+  std::vector<double> neum;            // (2,7)
+};
+
+bool dense_solve(int n, double *a, double *r);
+bool dense_inverse(int n, const double *a, double *inv);
+bool der_coeffs_generic(const double *h0, int cl, int der, int so, double imp[5], double exp_[7]);
+bool der_coeffs_global(const double *grid, int n, int der, int so, double *dl, double *dr);
+void pentalu(int n, double *dl);
+bool solver_coeffs_generic(const double *h0, int cl, int der, int bct, int so, double imp[5],
+                           double exp_[8]);
+bool dder_coeffs(const double *grid, int n, const int bct[2], int so, double *ddl, double *ddr);
+void solver_lhs_rhs(int n, const double *ddl, const double *ddr, double *lhs, double *rhs);
+bool extrapolation_coeffs(const double *grid, int n, const int bct[2], int so, double *neum);
+bool real_eig(int n, const double *A, double *wr, double *V, double *max_imag, const char **why);
+bool solver_setup_1d(const double *grid, int nall, const int bct[2], int so, Solver1D &out,
+                     std::string &err);
+
+// thread-local last error message (mf_last_error)
+int fail(int code, const std::string &msg);
+
+}  // namespace mf

@@ -1,0 +1,648 @@
+// Matrix-diagonalisation Helmholtz/Poisson solver on gfx950
+// (solve_3d, class_solver_3d.f90:83-299; solver_3d, class_field.f90:1072-1109).
+//
+// FP64 path (the reference algorithm):
+//   fold    R = g_int - sum_a inv(lhs_a) cl_a              (BC fold, :104-175)
+//   X/Y/Z   forward eigenvector transforms (MFMA FP64), divide fused into Z
+//   Z/Y/X   backward transforms, X scatters into u's interior
+//   faces   Dirichlet copy / Neumann extrapolation          (:261-297, :658-776)
+//
+// Mixed path: the same six transforms in FP32 on MFMA inside an iterative
+// refinement whose residual is taken in FP64 with the solver's own discrete
+// operator.  The operator sum_a inv(lhs_a) rhs_a + sigma is multiplied
+// through by P = lhs_x (x) lhs_y (x) lhs_z, which commutes with every term, so
+// the residual r' = P g_f - B u with
+//   B = rhs_x lhs_y lhs_z + lhs_x rhs_y lhs_z + lhs_x lhs_y rhs_z + sigma P
+// is a product of banded 1-D operators (no line solves), and the inner solve
+// uses W'_a = inv(V_a) inv(lhs_a) in place of inv(V_a), which absorbs P^-1.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gemm.h"
+#include "internal.h"
+#include "mflops/mf.h"
+
+namespace mf {
+
+// ---------------------------------------------------------------------------
+// Banded 1-D operator rows: row i acts on columns c0[i] .. c0[i]+6.
+// ---------------------------------------------------------------------------
+struct Band {
+  const int *c0;
+  const double *w;  // (n,8) row-major, 7 used
+};
+
+// BC fold: R(i,j,k) = g(i+1,j+1,k+1) - sum over the 6 faces of
+// fold_a(idx)*bc_a  with fold vectors fx1(i) = inv_lhs(i,1) ddr(1,1) +
+// inv_lhs(i,2) ddr(2,1), fx2(i) = inv_lhs(i,n-1) ddr(n-1,8) + inv_lhs(i,n) ddr(n,7).
+__global__ void __launch_bounds__(256)
+    fold_kernel(const double *__restrict__ g, double *__restrict__ R, int n1, int n2, int n3,
+                const double *__restrict__ fold,  // fx1 fx2 fy1 fy2 fz1 fz2
+                const double *__restrict__ bcx, const double *__restrict__ bcy,
+                const double *__restrict__ bcz) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long N = (long)n1 * n2 * n3;
+  if (p >= N) return;
+  const int i = (int)(p % n1);
+  const long jk = p / n1;
+  const int j = (int)(jk % n2), k = (int)(jk / n2);
+  const int nx = n1 + 2, ny = n2 + 2;
+  const double *fx1 = fold, *fx2 = fx1 + n1, *fy1 = fx2 + n1, *fy2 = fy1 + n2;
+  const double *fz1 = fy2 + n2, *fz2 = fz1 + n3;
+  double v = g[(i + 1) + (long)nx * ((j + 1) + (long)ny * (k + 1))];
+  const long sx = (long)n2 * n3, sy = (long)n1 * n3, sz = (long)n1 * n2;
+  v -= fx1[i] * bcx[j + (long)n2 * k];
+  v -= fx2[i] * bcx[sx + j + (long)n2 * k];
+  v -= fy1[j] * bcy[i + (long)n1 * k];
+  v -= fy2[j] * bcy[sy + i + (long)n1 * k];
+  v -= fz1[k] * bcz[i + (long)n1 * j];
+  v -= fz2[k] * bcz[sz + i + (long)n1 * j];
+  R[p] = v;
+}
+
+// Faces of u: Dirichlet copies bc; Neumann solves the one-sided 7-point
+// first-derivative formula for the boundary value
+// (class_solver_3d.f90:261-297, :658-776).  Edges are not touched.
+__global__ void __launch_bounds__(256)
+    faces_kernel(double *__restrict__ u, int n1, int n2, int n3, const int *__restrict__ bct,
+                 const double *__restrict__ neum,  // 3 x (2,7) column-major
+                 const double *__restrict__ bcx, const double *__restrict__ bcy,
+                 const double *__restrict__ bcz) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long fx = (long)n2 * n3, fy = (long)n1 * n3, fz = (long)n1 * n2;
+  const long nx = n1 + 2, ny = n2 + 2;
+  auto U = [&](long i, long j, long k) -> double & { return u[i + nx * (j + ny * k)]; };
+  int axis, side;
+  long a, b;  // interior face indices (0-based)
+  if (t < 2 * fx) { axis = 0; side = (int)(t / fx); long r = t % fx; a = r % n2; b = r / n2; }
+  else if ((t -= 2 * fx) < 2 * fy) { axis = 1; side = (int)(t / fy); long r = t % fy; a = r % n1; b = r / n1; }
+  else if ((t -= 2 * fy) < 2 * fz) { axis = 2; side = (int)(t / fz); long r = t % fz; a = r % n1; b = r / n1; }
+  else return;
+  const int code = bct[2 * axis + side];
+  const double *nm = neum + 14 * axis;  // nm[s + 2*m] = neum(s+1, m+1)
+  const int n = axis == 0 ? n1 : (axis == 1 ? n2 : n3);
+  const double *bcarr = axis == 0 ? bcx : (axis == 1 ? bcy : bcz);
+  const long fsz = axis == 0 ? fx : (axis == 1 ? fy : fz);
+  const long lenA = axis == 0 ? n2 : n1;
+  const double bcv = bcarr[side * fsz + a + lenA * b];
+  // position along the axis -> full-array index
+  auto at = [&](long q) -> double & {
+    if (axis == 0) return U(q, a + 1, b + 1);
+    if (axis == 1) return U(a + 1, q, b + 1);
+    return U(a + 1, b + 1, q);
+  };
+  if (code == 1) {
+    at(side ? n + 1 : 0) = bcv;
+  } else if (code == 2 || code == 3) {
+    if (side == 0) {
+      double acc = bcv;
+      for (int m = 1; m < 7; ++m) acc -= nm[0 + 2 * m] * at(m);
+      at(0) = acc * (1.0 / nm[0]);
+    } else {
+      double acc = bcv;
+      for (int m = 5; m >= 0; --m) acc -= nm[1 + 2 * m] * at(n - 5 + m);
+      at(n + 1) = acc * (1.0 / nm[1 + 2 * 6]);
+    }
+  }
+}
+
+// out(p) = sum_c band(row p_axis, c) in(p with axis index c), for all interior
+// points of an (n1,n2,n3) grid; inputs are strided (base + i*1 + j*ldj + k*ldk)
+// so the full-size u can be read in place.  Two bands on one input
+// (out1 = B1 in, out2 = B2 in) or a fused pair (out2 = B2 in + B3 in2).
+// FINAL: out1 = bprime - (B1 in + B3 in2), with ||out1||^2 accumulated.
+template <int AXIS, int MODE>
+__global__ void __launch_bounds__(256)
+    band_kernel(const double *__restrict__ in, long ldj, long ldk, const double *__restrict__ in2,
+                double *__restrict__ out1, double *__restrict__ out2, int n1, int n2, int n3,
+                Band B1, Band B2, Band B3, double s2, const double *__restrict__ bprime,
+                double *__restrict__ norm2) {
+  // MODE 0: out1 = B1 in
+  // MODE 1: out1 = B1 in ; out2 = B2 in + s2 * B1 in          (z-pass)
+  // MODE 2: out1 = B1 in ; out2 = B2 in + B1 in2              (y-pass)
+  // MODE 3: out1 = bprime - (B2 in + B1 in2), norm             (x-pass)
+  // MODE 4: out1 = B1 in, norm of out1                         (b' x-pass)
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long N = (long)n1 * n2 * n3;
+  double contrib = 0.0;
+  if (p < N) {
+    const int i = (int)(p % n1);
+    const long jk = p / n1;
+    const int j = (int)(jk % n2), k = (int)(jk / n2);
+    const int row = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+    const long stride = AXIS == 0 ? 1 : (AXIS == 1 ? ldj : ldk);
+    const long inbase = (AXIS == 0 ? 0 : i) + (AXIS == 1 ? 0 : (long)j * ldj) +
+                        (AXIS == 2 ? 0 : (long)k * ldk);
+    const long in2base = (AXIS == 0 ? 0 : i) + (AXIS == 1 ? 0 : (long)j * n1) +
+                         (AXIS == 2 ? 0 : (long)k * n1 * n2);
+    const long stride2 = AXIS == 0 ? 1 : (AXIS == 1 ? n1 : (long)n1 * n2);
+    const int c = B1.c0[row];
+    const double *w1 = B1.w + 8 * row;
+    double a1 = 0.0;
+    if (MODE == 3) {
+      // in = t2 (uses B2 = lhs_x), in2 = t1 (uses B1 = rhs_x); both interior-contiguous
+      const int c2 = B2.c0[row];
+      const double *w2 = B2.w + 8 * row;
+      double a2 = 0.0;
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        a1 = fma(w1[m], in2[in2base + (long)(c + m) * stride2], a1);
+        a2 = fma(w2[m], in[inbase + (long)(c2 + m) * stride], a2);
+      }
+      const double r = bprime[p] - (a1 + a2);
+      out1[p] = r;
+      contrib = r * r;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 7; ++m) a1 = fma(w1[m], in[inbase + (long)(c + m) * stride], a1);
+      if (MODE == 4) {
+        out1[p] = a1;
+        contrib = a1 * a1;
+      } else {
+        out1[p] = a1;
+      }
+      if (MODE == 1) {
+        const int c2 = B2.c0[row];
+        const double *w2 = B2.w + 8 * row;
+        double a2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) a2 = fma(w2[m], in[inbase + (long)(c2 + m) * stride], a2);
+        out2[p] = a2 + s2 * a1;
+      } else if (MODE == 2) {
+        const int c2 = B2.c0[row];
+        const double *w2 = B2.w + 8 * row;
+        double a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) {
+          a2 = fma(w2[m], in[inbase + (long)(c2 + m) * stride], a2);
+          a3 = fma(w1[m], in2[in2base + (long)(c + m) * stride2], a3);
+        }
+        out2[p] = a2 + a3;
+      }
+    }
+  }
+  if (MODE == 3 || MODE == 4) {
+    // wavefront reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(norm2, contrib);
+  }
+}
+
+}  // namespace mf
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+struct mf_solver3d {
+  int nx = 0, ny = 0, nz = 0, n1 = 0, n2 = 0, n3 = 0, so = 6;
+  int bct[6] = {1, 1, 1, 1, 1, 1};
+  mf::Solver1D ax[3];
+  // device memory
+  void *dmem = nullptr;
+  // FP64 eigen transforms: Wx = inv_vect_x, Vx = vect_x; Wy_t = inv_vect_y^T ...
+  double *Wx = nullptr, *Vx = nullptr, *Wyt = nullptr, *Vyt = nullptr, *Wzt = nullptr, *Vzt = nullptr;
+  float *Wx32 = nullptr, *Vx32 = nullptr, *Wyt32 = nullptr, *Vyt32 = nullptr, *Wzt32 = nullptr,
+        *Vzt32 = nullptr;  // mixed: W' = inv(V) inv(lhs)
+  double *lam = nullptr;   // lx(n1) ly(n2) lz(n3)
+  double *fold = nullptr;  // fx1 fx2 fy1 fy2 fz1 fz2
+  double *neum = nullptr;  // 3 x 14
+  int *bctd = nullptr;     // 6
+  int *bandc = nullptr;    // [axis][M|R] row starts
+  double *bandw = nullptr; // [axis][M|R] (n,8)
+  mf::Band M[3], R[3];
+  double *T1 = nullptr, *T2 = nullptr;  // interior FP64 work
+  double *T3 = nullptr, *T4 = nullptr, *T5 = nullptr;  // residual work (mixed)
+  float *F1 = nullptr, *F2 = nullptr;   // interior FP32 work (mixed)
+  double *norms = nullptr;              // device scalars
+  double *hnorms = nullptr;             // pinned host mirror
+};
+
+namespace {
+#define HIPCHK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return mf::fail(MF_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void transpose(int n, const std::vector<double> &a, std::vector<double> &t) {
+  t.resize((size_t)n * n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) t[j + (size_t)n * i] = a[i + (size_t)n * j];
+}
+
+// rows of a dense banded matrix as (start column, 7 weights)
+bool to_band(int n, const std::vector<double> &a, std::vector<int> &c0, std::vector<double> &w) {
+  c0.assign(n, 0);
+  w.assign((size_t)8 * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    int lo = n, hi = -1;
+    for (int j = 0; j < n; ++j)
+      if (a[i + (size_t)n * j] != 0.0) { lo = std::min(lo, j); hi = std::max(hi, j); }
+    if (hi < 0) { lo = hi = i; }
+    if (hi - lo >= 7) return false;
+    lo = std::min(lo, n - 7);  // keep the 7-wide window inside the line
+    lo = std::max(lo, 0);
+    if (hi - lo >= 7) return false;
+    c0[i] = lo;
+    for (int m = 0; m < 7; ++m) w[(size_t)8 * i + m] = a[i + (size_t)n * (lo + m)];
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int mf_solver_coeffs_1d(const double *grid, int nall, const int bct[2], int so,
+                                   double *ddl, double *ddr, double *eigen, double *vect,
+                                   double *inv_vect, double *inv_lhs, double *neum) {
+  if (!grid || !bct) return mf::fail(MF_ERR_ARG, "mf_solver_coeffs_1d: null pointer");
+  mf::Solver1D s;
+  std::string err;
+  if (!mf::solver_setup_1d(grid, nall, bct, so, s, err))
+    return mf::fail(err.find("at least") != std::string::npos || err.find("must") != std::string::npos
+                        ? MF_ERR_ARG
+                        : MF_ERR_SETUP,
+                    "mf_solver_coeffs_1d: " + err);
+  const size_t n = s.n;
+  if (ddl) std::memcpy(ddl, s.ddl.data(), sizeof(double) * 5 * n);
+  if (ddr) std::memcpy(ddr, s.ddr.data(), sizeof(double) * 8 * n);
+  if (eigen) std::memcpy(eigen, s.eigen.data(), sizeof(double) * n);
+  if (vect) std::memcpy(vect, s.vect.data(), sizeof(double) * n * n);
+  if (inv_vect) std::memcpy(inv_vect, s.inv_vect.data(), sizeof(double) * n * n);
+  if (inv_lhs) std::memcpy(inv_lhs, s.inv_lhs.data(), sizeof(double) * n * n);
+  if (neum) std::memcpy(neum, s.neum.data(), sizeof(double) * 14);
+  return MF_OK;
+}
+
+extern "C" int mf_solver3d_dims(const mf_solver3d *sc, int *n1, int *n2, int *n3) {
+  if (!sc) return mf::fail(MF_ERR_ARG, "mf_solver3d_dims: null handle");
+  if (n1) *n1 = sc->n1;
+  if (n2) *n2 = sc->n2;
+  if (n3) *n3 = sc->n3;
+  return MF_OK;
+}
+
+extern "C" int mf_solver3d_destroy(mf_solver3d *sc) {
+  if (!sc) return MF_OK;
+  if (sc->dmem) (void)hipFree(sc->dmem);
+  if (sc->hnorms) (void)hipHostFree(sc->hnorms);
+  delete sc;
+  return MF_OK;
+}
+
+extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, int ny,
+                                  const double *gz, int nz, const int bct[6], int so,
+                                  mf_solver3d **out) {
+  if (!gx || !gy || !gz || !bct || !out) return mf::fail(MF_ERR_ARG, "mf_solver3d_create: null pointer");
+  if (nx < 9 || ny < 9 || nz < 9)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_create: at least 9 points per direction are needed");
+  mf_solver3d *h = new mf_solver3d();
+  h->nx = nx; h->ny = ny; h->nz = nz;
+  h->n1 = nx - 2; h->n2 = ny - 2; h->n3 = nz - 2;
+  h->so = so;
+  std::memcpy(h->bct, bct, sizeof(int) * 6);
+  const double *grids[3] = {gx, gy, gz};
+  const int nall[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    std::string err;
+    if (!mf::solver_setup_1d(grids[a], nall[a], bct + 2 * a, so, h->ax[a], err)) {
+      delete h;
+      return mf::fail(err.find("at least") != std::string::npos || err.find("must") != std::string::npos
+                          ? MF_ERR_ARG
+                          : MF_ERR_SETUP,
+                      std::string("mf_solver3d_create: ") + "xyz"[a] + "-direction: " + err);
+    }
+  }
+  const int n[3] = {h->n1, h->n2, h->n3};
+  // host images of the device tables
+  std::vector<double> W[3], V[3], Wp[3];
+  for (int a = 0; a < 3; ++a) {
+    const mf::Solver1D &s = h->ax[a];
+    const int m = n[a];
+    // W' = inv(V) inv(lhs) for the mixed path
+    std::vector<double> wp((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j)
+      for (int k = 0; k < m; ++k) {
+        const double b = s.inv_lhs[k + (size_t)m * j];
+        if (b == 0.0) continue;
+        for (int i = 0; i < m; ++i) wp[i + (size_t)m * j] += s.inv_vect[i + (size_t)m * k] * b;
+      }
+    if (a == 0) {
+      W[a] = s.inv_vect;
+      V[a] = s.vect;
+      Wp[a] = wp;
+    } else {  // y and z act from the right: store transposes (class_solver_3d.f90:403-409)
+      transpose(m, s.inv_vect, W[a]);
+      transpose(m, s.vect, V[a]);
+      transpose(m, wp, Wp[a]);
+    }
+  }
+  std::vector<double> lam, fold, neum;
+  for (int a = 0; a < 3; ++a) lam.insert(lam.end(), h->ax[a].eigen.begin(), h->ax[a].eigen.end());
+  for (int a = 0; a < 3; ++a) {
+    const mf::Solver1D &s = h->ax[a];
+    const int m = n[a];
+    auto IL = [&](int i, int j) { return s.inv_lhs[i + (size_t)m * j]; };
+    auto DR = [&](int i, int c) { return s.ddr[i + (size_t)m * c]; };
+    std::vector<double> f1(m), f2(m);
+    for (int i = 0; i < m; ++i) {
+      f1[i] = IL(i, 0) * DR(0, 0) + IL(i, 1) * DR(1, 0);
+      f2[i] = IL(i, m - 2) * DR(m - 2, 7) + IL(i, m - 1) * DR(m - 1, 6);
+    }
+    fold.insert(fold.end(), f1.begin(), f1.end());
+    fold.insert(fold.end(), f2.begin(), f2.end());
+    neum.insert(neum.end(), s.neum.begin(), s.neum.end());
+  }
+  std::vector<int> bandc;
+  std::vector<double> bandw;
+  for (int a = 0; a < 3; ++a) {
+    std::vector<int> c;
+    std::vector<double> w;
+    if (!to_band(n[a], h->ax[a].lhs, c, w) ) {
+      delete h;
+      return mf::fail(MF_ERR_SETUP, "mf_solver3d_create: lhs is wider than 7 bands");
+    }
+    bandc.insert(bandc.end(), c.begin(), c.end());
+    bandw.insert(bandw.end(), w.begin(), w.end());
+    if (!to_band(n[a], h->ax[a].rhs, c, w)) {
+      delete h;
+      return mf::fail(MF_ERR_SETUP, "mf_solver3d_create: rhs is wider than 7 bands");
+    }
+    bandc.insert(bandc.end(), c.begin(), c.end());
+    bandw.insert(bandw.end(), w.begin(), w.end());
+  }
+  // device layout: one allocation carved into 256-byte aligned arrays; the
+  // carve runs twice, first to size the allocation, then to assign pointers
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  double *dW[3], *dV[3];
+  float *fW[3], *fV[3];
+  auto carve = [&](double *base) -> size_t {
+    size_t off = 0;
+    auto take = [&](size_t cnt) {
+      double *r = base ? base + off : nullptr;
+      off += (cnt + 31) / 32 * 32;
+      return r;
+    };
+    for (int a = 0; a < 3; ++a) { dW[a] = take((size_t)n[a] * n[a]); dV[a] = take((size_t)n[a] * n[a]); }
+    for (int a = 0; a < 3; ++a) {
+      fW[a] = (float *)take(((size_t)n[a] * n[a] + 1) / 2);
+      fV[a] = (float *)take(((size_t)n[a] * n[a] + 1) / 2);
+    }
+    h->lam = take(lam.size());
+    h->fold = take(fold.size());
+    h->neum = take(neum.size());
+    h->bandw = take(bandw.size());
+    h->norms = take(8);
+    h->T1 = take(N); h->T2 = take(N); h->T3 = take(N); h->T4 = take(N); h->T5 = take(N);
+    h->F1 = (float *)take((N + 1) / 2);
+    h->F2 = (float *)take((N + 1) / 2);
+    h->bandc = (int *)take((bandc.size() + 1) / 2);
+    h->bctd = (int *)take(4);
+    return off;
+  };
+  const size_t nd = carve(nullptr);
+  hipError_t e = hipMalloc(&h->dmem, nd * sizeof(double));
+  if (e != hipSuccess) {
+    delete h;
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_create: hipMalloc: ") + hipGetErrorString(e));
+  }
+  carve((double *)h->dmem);
+  h->Wx = dW[0]; h->Vx = dV[0]; h->Wyt = dW[1]; h->Vyt = dV[1]; h->Wzt = dW[2]; h->Vzt = dV[2];
+  h->Wx32 = fW[0]; h->Vx32 = fV[0]; h->Wyt32 = fW[1]; h->Vyt32 = fV[1]; h->Wzt32 = fW[2]; h->Vzt32 = fV[2];
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  std::vector<float> tmp;
+  auto upf = [&](float *dst, const std::vector<double> &src) {
+    tmp.assign(src.begin(), src.end());
+    return up(dst, tmp.data(), tmp.size() * sizeof(float));
+  };
+  e = hipSuccess;
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    e = up(dW[a], W[a].data(), W[a].size() * sizeof(double));
+    if (e == hipSuccess) e = up(dV[a], V[a].data(), V[a].size() * sizeof(double));
+    if (e == hipSuccess) e = upf(fW[a], Wp[a]);
+    if (e == hipSuccess) e = upf(fV[a], V[a]);
+  }
+  if (e == hipSuccess) e = up(h->lam, lam.data(), lam.size() * sizeof(double));
+  if (e == hipSuccess) e = up(h->fold, fold.data(), fold.size() * sizeof(double));
+  if (e == hipSuccess) e = up(h->neum, neum.data(), neum.size() * sizeof(double));
+  if (e == hipSuccess) e = up(h->bandw, bandw.data(), bandw.size() * sizeof(double));
+  if (e == hipSuccess) e = up(h->bandc, bandc.data(), bandc.size() * sizeof(int));
+  if (e == hipSuccess) e = up(h->bctd, h->bct, 6 * sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc((void **)&h->hnorms, 8 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    mf_solver3d_destroy(h);
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_create: upload: ") + hipGetErrorString(e));
+  }
+  size_t coff = 0, woff = 0;
+  for (int a = 0; a < 3; ++a) {
+    h->M[a].c0 = h->bandc + coff; h->M[a].w = h->bandw + woff;
+    coff += n[a]; woff += (size_t)8 * n[a];
+    h->R[a].c0 = h->bandc + coff; h->R[a].w = h->bandw + woff;
+    coff += n[a]; woff += (size_t)8 * n[a];
+  }
+  *out = h;
+  return MF_OK;
+}
+
+namespace {
+using mf::EpiArgs;
+
+template <typename T, typename TA, typename TB, typename TC, int BM, int BN, int EPI>
+void launch_gemm(const TA *A, long lda, long sA, const TB *B, long ldb, TC *C, long ldc, long sC,
+                 int M, int N, int K, int batch, const EpiArgs &ep, hipStream_t st) {
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  hipLaunchKernelGGL((mf::gemm_mode_kernel<T, TA, TB, TC, BM, BN, EPI>), grid, dim3(256), 0, st, A,
+                     lda, sA, B, ldb, C, ldc, sC, M, N, K, ep);
+}
+
+// Six transforms: in (interior, type TIn) -> u interior (scatter or add).
+template <typename T, typename TIn>
+void transforms(mf_solver3d *h, const TIn *in, T *w1, T *w2, const T *Wx, const T *Wyt,
+                const T *Wzt, const T *Vx, const T *Vyt, const T *Vzt, double sigma, double *u,
+                bool accumulate, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  const long n12 = (long)n1 * n2;
+  EpiArgs ep;
+  std::memset(&ep, 0, sizeof(ep));
+  ep.lx = h->lam;
+  ep.ly = h->lam + n1;
+  ep.lz = h->lam + n1 + n2;
+  ep.sigma = sigma;
+  ep.n1 = n1;
+  ep.n2 = n2;
+  ep.u = u;
+  ep.nx = h->nx;
+  ep.ny = h->ny;
+  // forward X: w1 = Wx . in
+  launch_gemm<T, T, TIn, T, 256, 64, mf::EPI_STORE>(Wx, n1, 0, in, n1, w1, n1, 0, n1, n2 * n3, n1, 1, ep, st);
+  // forward Y: w2_k = w1_k . Wyt
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(w1, n1, n12, Wyt, n2, w2, n1, n12, n1, n2, n2, n3, ep, st);
+  // forward Z with the eigenvalue divide: w1 = (w2 . Wzt) / lambda
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_DIVIDE>(w2, n12, 0, Wzt, n3, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  // backward Z: w2 = w1 . Vzt
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(w1, n12, 0, Vzt, n3, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  // backward Y: w1_k = w2_k . Vyt
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(w2, n1, n12, Vyt, n2, w1, n1, n12, n1, n2, n2, n3, ep, st);
+  // backward X into u's interior
+  if (accumulate)
+    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER_ADD>(Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+  else
+    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER>(Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+}
+
+int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy,
+                 const double *bcz, hipStream_t st) {
+  const long nf = 2 * ((long)h->n2 * h->n3 + (long)h->n1 * h->n3 + (long)h->n1 * h->n2);
+  hipLaunchKernelGGL(mf::faces_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, u,
+                     h->n1, h->n2, h->n3, h->bctd, h->neum, bcx, bcy, bcz);
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+// r' = b' - B u with B the homogeneous operator in multiplied-through form;
+// u is read in place from the full array (interior at offset (1,1,1)).
+// Work: T3 = s1, T4 = s2 (z-pass), T5 = t1, T1 = t2 (y-pass); r' -> T3.
+int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime, double *rp,
+             double *norm2_dev, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  const long N = (long)n1 * n2 * n3;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  const long ldj = h->nx, ldk = (long)h->nx * h->ny;
+  const double *u0 = u + 1 + ldj + ldk;
+  const long il = n1, ik = (long)n1 * n2;
+  mf::Band none = {nullptr, nullptr};
+  const double *dnull = nullptr;
+  double *wnull = nullptr;
+  hipLaunchKernelGGL((mf::band_kernel<2, 1>), dim3(blocks), dim3(256), 0, st, u0, ldj, ldk, dnull,
+                     h->T3, h->T4, n1, n2, n3, h->M[2], h->R[2], none, sigma, dnull, wnull);
+  hipLaunchKernelGGL((mf::band_kernel<1, 2>), dim3(blocks), dim3(256), 0, st, (const double *)h->T3,
+                     il, ik, (const double *)h->T4, h->T5, h->T1, n1, n2, n3, h->M[1], h->R[1], none,
+                     0.0, dnull, wnull);
+  hipLaunchKernelGGL((mf::band_kernel<0, 3>), dim3(blocks), dim3(256), 0, st, (const double *)h->T1,
+                     il, ik, (const double *)h->T5, rp, wnull, n1, n2, n3, h->R[0], h->M[0], none,
+                     0.0, bprime, norm2_dev);
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+// b' = lhs_x lhs_y lhs_z g_f with ||b'||^2; g_f in T1, b' -> T2.
+int multiply_through(mf_solver3d *h, double *norm2_dev, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  const long N = (long)n1 * n2 * n3;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  const long il = n1, ik = (long)n1 * n2;
+  mf::Band none = {nullptr, nullptr};
+  const double *dnull = nullptr;
+  double *wnull = nullptr;
+  hipLaunchKernelGGL((mf::band_kernel<2, 0>), dim3(blocks), dim3(256), 0, st, (const double *)h->T1,
+                     il, ik, dnull, h->T3, wnull, n1, n2, n3, h->M[2], none, none, 0.0, dnull, wnull);
+  hipLaunchKernelGGL((mf::band_kernel<1, 0>), dim3(blocks), dim3(256), 0, st, (const double *)h->T3,
+                     il, ik, dnull, h->T4, wnull, n1, n2, n3, h->M[1], none, none, 0.0, dnull, wnull);
+  hipLaunchKernelGGL((mf::band_kernel<0, 4>), dim3(blocks), dim3(256), 0, st, (const double *)h->T4,
+                     il, ik, dnull, h->T2, wnull, n1, n2, n3, h->M[0], none, none, 0.0, dnull,
+                     norm2_dev);
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+}  // namespace
+
+extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, double *u,
+                                 const double *bcx, const double *bcy, const double *bcz,
+                                 int precision, double tol, int max_iter, mf_solve_report *rep,
+                                 void *stream) {
+  if (!h || !g || !u || !bcx || !bcy || !bcz)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: null pointer");
+  if (precision != MF_FP64 && precision != MF_MIXED)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: precision must be MF_FP64 or MF_MIXED");
+  if (g == u) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: g and u must not alias");
+  if (!std::isfinite(sigma)) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: sigma is not finite");
+  hipStream_t st = (hipStream_t)stream;
+  const long N = (long)h->n1 * h->n2 * h->n3;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  if (rep) { rep->iterations = 0; rep->residual = 0.0; rep->residual0 = 0.0; }
+  if (precision == MF_FP64) {
+    hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T2, h->n1, h->n2,
+                       h->n3, (const double *)h->fold, bcx, bcy, bcz);
+    transforms<double, double>(h, h->T2, h->T1, h->T2, h->Wx, h->Wyt, h->Wzt, h->Vx, h->Vyt,
+                               h->Vzt, sigma, u, false, st);
+    HIPCHK(hipGetLastError());
+    return launch_faces(h, u, bcx, bcy, bcz, st);
+  }
+  // ---- mixed precision: FP32 MFMA transforms, FP64 residual refinement
+  if (!(tol > 0.0)) tol = 1e-12;
+  if (max_iter <= 0) max_iter = 8;
+  hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T1, h->n1, h->n2,
+                     h->n3, (const double *)h->fold, bcx, bcy, bcz);
+  HIPCHK(hipMemsetAsync(h->norms, 0, 8 * sizeof(double), st));
+  int rc = multiply_through(h, h->norms + 0, st);
+  if (rc != MF_OK) return rc;
+  // first low-precision solve: u_int = S(b')
+  transforms<float, double>(h, h->T2, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
+                            h->Vyt32, h->Vzt32, sigma, u, false, st);
+  int it = 1;
+  double rel = 0.0;
+  for (;;) {
+    HIPCHK(hipMemsetAsync(h->norms + 1, 0, sizeof(double), st));
+    rc = residual(h, u, sigma, h->T2, h->T3, h->norms + 1, st);
+    if (rc != MF_OK) return rc;
+    HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double b2 = h->hnorms[0], r2 = h->hnorms[1];
+    rel = (b2 > 0.0) ? std::sqrt(r2 / b2) : std::sqrt(r2);
+    if (rep) {
+      if (it == 1) rep->residual0 = rel;
+      rep->residual = rel;
+      rep->iterations = it;
+    }
+    if (!(rel > tol) || it >= max_iter) break;
+    transforms<float, double>(h, h->T3, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
+                              h->Vyt32, h->Vzt32, sigma, u, true, st);
+    ++it;
+  }
+  rc = launch_faces(h, u, bcx, bcy, bcz, st);
+  if (rc != MF_OK) return rc;
+  if (!(rel <= tol))
+    return mf::fail(MF_ERR_NOCONV, "mf_solver3d_solve: refinement stopped at relative residual " +
+                                       std::to_string(rel));
+  return MF_OK;
+}
+
+extern "C" int mf_solver3d_solve_host(mf_solver3d *h, const double *g, double sigma, double *u,
+                                      const double *bcx, const double *bcy, const double *bcz,
+                                      int precision, double tol, int max_iter,
+                                      mf_solve_report *rep) {
+  if (!h || !g || !u || !bcx || !bcy || !bcz)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_solve_host: null pointer");
+  const size_t full = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+  const size_t bx = (size_t)2 * h->n2 * h->n3 * sizeof(double);
+  const size_t by = (size_t)2 * h->n1 * h->n3 * sizeof(double);
+  const size_t bz = (size_t)2 * h->n1 * h->n2 * sizeof(double);
+  char *d = nullptr;
+  HIPCHK(hipMalloc(&d, 2 * full + bx + by + bz));
+  double *dg = (double *)d, *du = (double *)(d + full);
+  double *dbx = (double *)(d + 2 * full), *dby = (double *)(d + 2 * full + bx);
+  double *dbz = (double *)(d + 2 * full + bx + by);
+  hipError_t e = hipMemcpy(dg, g, full, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(du, u, full, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dbx, bcx, bx, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dby, bcy, by, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dbz, bcz, bz, hipMemcpyHostToDevice);
+  int rc = MF_OK;
+  if (e == hipSuccess) {
+    rc = mf_solver3d_solve(h, dg, sigma, du, dbx, dby, dbz, precision, tol, max_iter, rep, nullptr);
+    if (rc == MF_OK || rc == MF_ERR_NOCONV) {
+      hipError_t e2 = hipMemcpy(u, du, full, hipMemcpyDeviceToHost);
+      if (e2 != hipSuccess) e = e2;
+    }
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_solve_host: ") + hipGetErrorString(e));
+  return rc;
+}

@@ -1,0 +1,292 @@
+"""mflops: MI355X-native compact derivatives and diagonalisation solver.
+
+Python mirror of the reference's module API for this path, over the C ABI
+(include/mflops/mf.h) of the HIP library.  Device memory and streams come from
+PyTorch; every computation runs in libmflops.so.
+
+Reference interface -> here (paths relative to the reference root):
+  der_coeffs_init          src/class_derivatives/class_derivatives.f90:111 -> DerivativeCoefficients
+  der, dder, der_s, dder_s src/class_derivatives/class_derivatives.f90:186-260 -> .der/.dder/.der_s/.dder_s
+  derx .. dderz            src/class_field/class_field.f90:629-978 -> derx .. dderz
+  solver_coeffs_init_3d    src/class_solver/class_solver_3d.f90:380 -> SolverCoeffs3D
+  solver_3d                src/class_field/class_field.f90:1072 -> solver_3d
+  type(boundary_condition) src/class_field/class_field.f90:25-30 -> BoundaryCondition
+
+Fields are FP64 tensors of shape (nx, ny, nz) with Fortran strides
+(1, nx, nx*ny), i.e. the reference's column-major f(nx,ny,nz); use
+``fortran_empty``/``to_device``/``to_host`` to make and move them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import MF_FP64, MF_MIXED, MFError, SolveReport, check, load  # noqa: F401
+
+__all__ = [
+    "load", "MFError", "MF_FP64", "MF_MIXED", "SolveReport",
+    "fortran_empty", "fortran_zeros", "to_device", "to_host", "is_fortran",
+    "DerivativeCoefficients", "derivative", "derx", "dderx", "dery", "ddery", "derz", "dderz",
+    "BoundaryCondition", "SolverCoeffs3D", "solver_3d", "der_coeffs_1d", "solver_coeffs_1d",
+]
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+# ---------------------------------------------------------------------------
+# Fortran-ordered device fields
+# ---------------------------------------------------------------------------
+
+def fortran_empty(shape, device="cuda", dtype=None):
+    torch = _torch()
+    dtype = dtype or torch.float64
+    rev = tuple(reversed(tuple(shape)))
+    return torch.empty(rev, device=device, dtype=dtype).permute(*reversed(range(len(shape))))
+
+
+def fortran_zeros(shape, device="cuda", dtype=None):
+    t = fortran_empty(shape, device, dtype)
+    t.zero_()
+    return t
+
+
+def is_fortran(t) -> bool:
+    exp, s = [], 1
+    for d in t.shape:
+        exp.append(s)
+        s *= d
+    return all(st == e for st, e, d in zip(t.stride(), exp, t.shape) if d > 1)
+
+
+def to_device(a: np.ndarray, device="cuda"):
+    torch = _torch()
+    a = np.asarray(a, dtype=np.float64)
+    flat = torch.from_numpy(np.ascontiguousarray(a.ravel(order="F"))).to(device)
+    return flat.view(tuple(reversed(a.shape))).permute(*reversed(range(a.ndim)))
+
+
+def to_host(t) -> np.ndarray:
+    torch = _torch()
+    flat = t.permute(*reversed(range(t.dim()))).contiguous().cpu()
+    return np.asfortranarray(flat.numpy().reshape(tuple(reversed(t.shape))).transpose())
+
+
+def _ptr(t) -> int:
+    if not is_fortran(t):
+        raise ValueError("tensor must have Fortran strides (use fortran_empty/to_device)")
+    if t.dtype != _torch().float64:
+        raise TypeError("tensor must be float64")
+    return t.data_ptr()
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Host-side coefficient tables
+# ---------------------------------------------------------------------------
+
+def der_coeffs_1d(grid, der: int, so: int = 6):
+    """(dl_lu (n,5), dr (n,7)) exactly as der_coeffs_generic_gobal + pentalu."""
+    lib = load()
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    n = g.shape[0]
+    dl = np.zeros((n, 5), order="F")
+    dr = np.zeros((n, 7), order="F")
+    check(lib.mf_der_coeffs_1d(_dptr(g), n, der, so, _dptr(dl), _dptr(dr)))
+    return dl, dr
+
+
+def solver_coeffs_1d(grid, bct, so: int = 6) -> dict:
+    """One direction of solver_coeffs_init (class_solver_3d.f90:413-466)."""
+    lib = load()
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    nall = g.shape[0]
+    n = nall - 2
+    out = {
+        "ddl": np.zeros((n, 5), order="F"), "ddr": np.zeros((n, 8), order="F"),
+        "eigen": np.zeros(n), "vect": np.zeros((n, n), order="F"),
+        "inv_vect": np.zeros((n, n), order="F"), "inv_lhs": np.zeros((n, n), order="F"),
+        "neum": np.zeros((2, 7), order="F"),
+    }
+    b = (C.c_int * 2)(*[int(v) for v in bct])
+    check(lib.mf_solver_coeffs_1d(_dptr(g), nall, C.cast(b, C.c_void_p), so,
+                                  *[_dptr(out[k]) for k in
+                                    ("ddl", "ddr", "eigen", "vect", "inv_vect", "inv_lhs", "neum")]))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Derivatives
+# ---------------------------------------------------------------------------
+
+class DerivativeCoefficients:
+    """type(derivatives_coefficients) built by der_coeffs_init(grid, dc, n, solve, so)."""
+
+    def __init__(self, grid, solver: bool = False, so: int = 6):
+        self._lib = load()
+        g = np.ascontiguousarray(grid, dtype=np.float64)
+        self.n = int(g.shape[0])
+        self.so = so
+        self.solver = bool(solver)
+        h = C.c_void_p()
+        check(self._lib.mf_der_create(_dptr(g), self.n, int(self.solver), so, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mf_der_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _lines(self, x, order: int, sub: bool, out=None, stream=None):
+        torch = _torch()
+        if x.dim() == 1:
+            x2 = x.reshape(self.n, 1)
+        else:
+            x2 = x
+        if x2.shape[0] != self.n:
+            raise ValueError("line length %d != handle size %d" % (x2.shape[0], self.n))
+        nl = x2.shape[1]
+        if out is None:
+            out = torch.zeros_like(x) if x.dim() == 1 else fortran_zeros(tuple(x2.shape), x.device)
+        o2 = out.reshape(self.n, 1) if out.dim() == 1 else out
+        ld = x2.stride(1) if nl > 1 else self.n
+        if x2.stride(0) != 1 or (nl > 1 and (o2.stride(0) != 1 or o2.stride(1) != ld)):
+            raise ValueError("lines must be contiguous columns with a common stride")
+        check(self._lib.mf_der_lines(self._h, order, int(sub), C.c_void_p(x2.data_ptr()),
+                                     C.c_void_p(o2.data_ptr()), nl, ld, _stream(stream)))
+        return out
+
+    # class_derivatives.f90:186-260
+    def der(self, x, out=None):
+        return self._lines(x, 1, False, out)
+
+    def dder(self, x, out=None):
+        return self._lines(x, 2, False, out)
+
+    def der_s(self, x, out=None):
+        return self._lines(x, 1, True, out)
+
+    def dder_s(self, x, out=None):
+        return self._lines(x, 2, True, out)
+
+
+def derivative(dc: DerivativeCoefficients, x, axis: int, order: int, out=None, stream=None):
+    """3-D derivative along axis (class_field.f90:629-978); returns a new field."""
+    nx, ny, nz = x.shape
+    if out is None:
+        out = fortran_empty((nx, ny, nz), x.device)
+    check(dc._lib.mf_der3d(dc.handle, axis, order, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(out)),
+                           nx, ny, nz, _stream(stream)))
+    return out
+
+
+def derx(dc, x, out=None):
+    return derivative(dc, x, 0, 1, out)
+
+
+def dderx(dc, x, out=None):
+    return derivative(dc, x, 0, 2, out)
+
+
+def dery(dc, x, out=None):
+    return derivative(dc, x, 1, 1, out)
+
+
+def ddery(dc, x, out=None):
+    return derivative(dc, x, 1, 2, out)
+
+
+def derz(dc, x, out=None):
+    return derivative(dc, x, 2, 1, out)
+
+
+def dderz(dc, x, out=None):
+    return derivative(dc, x, 2, 2, out)
+
+
+# ---------------------------------------------------------------------------
+# Solver
+# ---------------------------------------------------------------------------
+
+@dataclass
+class BoundaryCondition:
+    """type(boundary_condition): bcx(ny-2,nz-2,2), bcy(nx-2,nz-2,2), bcz(nx-2,ny-2,2)."""
+    bcx: object
+    bcy: object
+    bcz: object
+
+    @staticmethod
+    def zeros(nx, ny, nz, device="cuda"):
+        return BoundaryCondition(fortran_zeros((ny - 2, nz - 2, 2), device),
+                                 fortran_zeros((nx - 2, nz - 2, 2), device),
+                                 fortran_zeros((nx - 2, ny - 2, 2), device))
+
+
+class SolverCoeffs3D:
+    """type(solver_coeffs_3d) built by solver_coeffs_init_3d(grid1,grid2,grid3,sc,n1,n2,n3,bct,so)."""
+
+    def __init__(self, gx, gy, gz, bct, so: int = 6):
+        self._lib = load()
+        gs = [np.ascontiguousarray(g, dtype=np.float64) for g in (gx, gy, gz)]
+        self.shape = tuple(int(g.shape[0]) for g in gs)
+        self.bct = tuple(int(b) for b in bct)
+        self.so = so
+        b = (C.c_int * 6)(*self.bct)
+        h = C.c_void_p()
+        check(self._lib.mf_solver3d_create(_dptr(gs[0]), self.shape[0], _dptr(gs[1]), self.shape[1],
+                                           _dptr(gs[2]), self.shape[2], C.cast(b, C.c_void_p), so,
+                                           C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mf_solver3d_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def solve(self, g, u, bc: BoundaryCondition, sigma: float, precision: int = MF_FP64,
+              tol: float = 1e-12, max_iter: int = 8, stream=None) -> SolveReport:
+        """solver_3d(sc, g, u, bc, sigma): u's interior and faces are overwritten."""
+        if tuple(g.shape) != self.shape or tuple(u.shape) != self.shape:
+            raise ValueError("field shape %s/%s != solver shape %s" % (g.shape, u.shape, self.shape))
+        nx, ny, nz = self.shape
+        for t, s in ((bc.bcx, (ny - 2, nz - 2, 2)), (bc.bcy, (nx - 2, nz - 2, 2)),
+                     (bc.bcz, (nx - 2, ny - 2, 2))):
+            if tuple(t.shape) != s:
+                raise ValueError("boundary array shape %s != %s" % (tuple(t.shape), s))
+        rep = SolveReport()
+        code = self._lib.mf_solver3d_solve(
+            self._h, C.c_void_p(_ptr(g)), float(sigma), C.c_void_p(_ptr(u)),
+            C.c_void_p(_ptr(bc.bcx)), C.c_void_p(_ptr(bc.bcy)), C.c_void_p(_ptr(bc.bcz)),
+            int(precision), float(tol), int(max_iter), C.byref(rep), _stream(stream))
+        check(code)
+        return rep
+
+
+def solver_3d(sc: SolverCoeffs3D, g, u, bc: BoundaryCondition, sigma: float,
+              precision: int = MF_FP64, **kw) -> SolveReport:
+    """solver_3d (class_field.f90:1072-1109)."""
+    return sc.solve(g, u, bc, sigma, precision, **kw)

@@ -1,0 +1,72 @@
+"""ctypes binding of libmflops.so (the C ABI declared in include/mflops/mf.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmflops.so")
+
+MF_OK, MF_ERR_ARG, MF_ERR_SETUP, MF_ERR_DEVICE, MF_ERR_NOCONV = 0, 1, 2, 3, 4
+MF_FP64, MF_MIXED = 0, 1
+
+# every exported symbol with (restype, argtypes)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("residual0", C.c_double)]
+
+
+SIGNATURES = {
+    "mf_last_error": (C.c_char_p, []),
+    "mf_version": (C.c_int, []),
+    "mf_der_coeffs_1d": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "mf_solver_coeffs_1d": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mf_der_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "mf_der_destroy": (C.c_int, [_vp]),
+    "mf_der_lines": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_longlong, _vp]),
+    "mf_der3d": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "mf_der3d_host": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "mf_solver3d_create": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int,
+                                     C.POINTER(_vp)]),
+    "mf_solver3d_destroy": (C.c_int, [_vp]),
+    "mf_solver3d_solve": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp, _vp, _vp, C.c_int,
+                                    C.c_double, C.c_int, C.POINTER(SolveReport), _vp]),
+    "mf_solver3d_solve_host": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp, _vp, _vp, C.c_int,
+                                         C.c_double, C.c_int, C.POINTER(SolveReport)]),
+    "mf_solver3d_dims": (C.c_int, [_vp, _ip, _ip, _ip]),
+}
+
+_lib = None
+
+
+class MFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("mflops error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmflops.so not built: run `python mflops3d-md_amd/build.py` "
+                           "(expected at %s)" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != MF_OK:
+        msg = load().mf_last_error()
+        raise MFError(code, msg.decode() if msg else "")
