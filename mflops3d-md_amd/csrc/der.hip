@@ -24,7 +24,7 @@ namespace mf {
 
 struct LineTab {      // device view of one packed scheme (dl LU + dr)
   const double *dr;   // (n,7) row-major: dr[7*i + m]
-  const double *p, *f, *d, *e, *ic;  // (n) each; ic = 1/c
+  const double *p, *f, *d, *e, *c;  // (n) each
   double alpha, beta;
   int n;
 };
@@ -109,9 +109,9 @@ __device__ __forceinline__ void solve_line(const double *__restrict__ x, double 
     for (int m = 0; m < 7; ++m) rr = fma(c[m], v[m], rr);
     yi = rr - T.p[n - 1] * ym2 - T.f[n - 1] * ym1 - T.beta * ym3;
     // ---- backward sweep
-    double xp1 = yi * T.ic[n - 1];                                   // x[n-1]
+    double xp1 = yi / T.c[n - 1];                                    // x[n-1]
     y[(long)(n - 1) * s] = xp1;
-    double xc = (ym1 - T.d[n - 2] * xp1) * T.ic[n - 2];              // x[n-2]
+    double xc = (ym1 - T.d[n - 2] * xp1) / T.c[n - 2];               // x[n-2]
     y[(long)(n - 2) * s] = xc;
     double xp2 = xp1;
     xp1 = xc;
@@ -133,7 +133,7 @@ __device__ __forceinline__ void solve_line(const double *__restrict__ x, double 
       for (int r = 0; r < CH; ++r) {
         const int i = j0 - r;
         if (i >= 1) {
-          double xi = (yb[r] - T.e[i] * xp2 - T.d[i] * xp1) * T.ic[i];
+          double xi = (yb[r] - T.e[i] * xp2 - T.d[i] * xp1) / T.c[i];
           y[(long)i * s] = xi;
           xp2 = xp1;
           xp1 = xi;
@@ -146,7 +146,7 @@ __device__ __forceinline__ void solve_line(const double *__restrict__ x, double 
       for (int r = 0; r < CH; ++r) yb[r] = nyb[r];
     }
     const double y0 = y[0];
-    y[0] = (y0 - T.e[0] * x2 - T.d[0] * x1 - T.alpha * x3) * T.ic[0];
+    y[0] = (y0 - T.e[0] * x2 - T.d[0] * x1 - T.alpha * x3) / T.c[0];
   }
 }
 
@@ -217,7 +217,7 @@ struct mf_der {
 };
 
 namespace {
-// pack one scheme: dr row-major (7n) | p | f | d | e | ic
+// pack one scheme: dr row-major (7n) | p | f | d | e | c
 void pack_scheme(int n, const std::vector<double> &dl, const std::vector<double> &dr,
                  std::vector<double> &dst) {
   dst.assign((size_t)12 * n, 0.0);
@@ -229,7 +229,7 @@ void pack_scheme(int n, const std::vector<double> &dl, const std::vector<double>
     p[n + i] = dl[n + i];
     p[2 * n + i] = dl[3 * (size_t)n + i];
     p[3 * n + i] = dl[4 * (size_t)n + i];
-    p[4 * n + i] = 1.0 / dl[2 * (size_t)n + i];
+    p[4 * n + i] = dl[2 * (size_t)n + i];
   }
 }
 #define HIPCHK(call)                                                             \
@@ -304,7 +304,7 @@ extern "C" int mf_der_create(const double *grid, int n, int solver_mode, int so,
       T.f = T.p + m;
       T.d = T.f + m;
       T.e = T.d + m;
-      T.ic = T.e + m;
+      T.c = T.e + m;
       T.alpha = h->host[o][sub][0][0];                       // dl(1,1)
       T.beta = h->host[o][sub][0][4 * (size_t)m + m - 1];    // dl(m,5) after pentalu
       T.n = m;

@@ -23,12 +23,17 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// Operand layout (both types): lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Output layout differs: result r of lane l is D[row(l,r)][l%16] with
+// row = l/16 + 4r for the f64 instruction and 4(l/16) + r for the f32 one
+// (measured on gfx950).
 template <>
 struct Mfma<double> {
   typedef d4_t acc_t;
   static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -36,6 +41,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 enum EpiMode { EPI_STORE = 0, EPI_DIVIDE = 1, EPI_SCATTER = 2, EPI_SCATTER_ADD = 3 };
@@ -126,7 +132,7 @@ __global__ void __launch_bounds__(256)
         for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Mfma<T>::op(bf[ni], af[mi], acc[mi][ni]);
     }
   }
-  // epilogue: lane holds C(m0+wm+16mi+lane%16, n0+wn+16ni+4(lane/16)+r)
+  // epilogue: lane holds C(m0+wm+16mi+lane%16, n0+wn+16ni+row(lane,r))
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
     const long m = m0 + wm + mi * 16 + (lane & 15);
@@ -143,7 +149,7 @@ __global__ void __launch_bounds__(256)
     for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long n = n0 + wn + ni * 16 + 4 * (lane >> 4) + r;
+        const long n = n0 + wn + ni * 16 + Mfma<T>::row(lane, r);
         if (n >= N) continue;
         const double v = (double)acc[mi][ni][r];
         if (EPI == EPI_STORE) {

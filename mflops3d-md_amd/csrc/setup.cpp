@@ -105,19 +105,58 @@ bool dense_inverse(int n, const double *a_in, double *inv) {
 // (class_derivatives_coefficient.f90:135-151, class_solver_coefficient_3d.f90:176-192):
 // implicit column j: a(der+1,j) = -1, a(i,j) = a(i-1,j) h(j)/(i-der-1);
 // explicit column j: a(1,j) = 1,     a(i,j) = a(i-1,j) h(j)/(i-1).
+//
+// These small systems are badly conditioned for the high-order second
+// derivative (the spread of an FP64 LU across LAPACK builds reaches 5e-11
+// relative for so=8), so they are built and solved in x87 80-bit extended
+// precision and rounded once to FP64.  The elimination order is fixed
+// (partial pivoting on the first maximal |a(i,k)|, row updates left to
+// right) so that any restatement using the same steps gets the same bits.
 // ---------------------------------------------------------------------------
-static void taylor_matrix(const double *h, int n1, int nt, int der, double *a) {
-  std::fill(a, a + (size_t)nt * nt, 0.0);
+typedef long double xreal;
+
+static void taylor_matrix(const double *h, int n1, int nt, int der, xreal *a) {
+  std::fill(a, a + (size_t)nt * nt, (xreal)0);
   for (int j = 0; j < n1; ++j) {
-    a[der + (size_t)nt * j] = -1.0;
+    a[der + (size_t)nt * j] = -1;
     for (int i = der + 1; i < nt; ++i)
-      a[i + (size_t)nt * j] = a[i - 1 + (size_t)nt * j] * h[j] / (double)(i - der);
+      a[i + (size_t)nt * j] = a[i - 1 + (size_t)nt * j] * (xreal)h[j] / (xreal)(i - der);
   }
   for (int j = n1; j < nt; ++j) {
-    a[(size_t)nt * j] = 1.0;
+    a[(size_t)nt * j] = 1;
     for (int i = 1; i < nt; ++i)
-      a[i + (size_t)nt * j] = a[i - 1 + (size_t)nt * j] * h[j] / (double)i;
+      a[i + (size_t)nt * j] = a[i - 1 + (size_t)nt * j] * (xreal)h[j] / (xreal)i;
   }
+}
+
+// Gaussian elimination with partial pivoting in extended precision; a is
+// column-major nt x nt, r the right-hand side; the solution is rounded to
+// FP64 into out.
+static bool taylor_solve(int nt, xreal *a, xreal *r, double *out) {
+  auto A = [&](int i, int j) -> xreal & { return a[i + (size_t)nt * j]; };
+  for (int k = 0; k < nt; ++k) {
+    int p = k;
+    xreal best = std::fabs(A(k, k));
+    for (int i = k + 1; i < nt; ++i)
+      if (std::fabs(A(i, k)) > best) { best = std::fabs(A(i, k)); p = i; }
+    if (best == 0) return false;
+    if (p != k) {
+      for (int j = 0; j < nt; ++j) std::swap(A(k, j), A(p, j));
+      std::swap(r[k], r[p]);
+    }
+    for (int i = k + 1; i < nt; ++i) {
+      const xreal l = A(i, k) / A(k, k);
+      for (int j = k; j < nt; ++j) A(i, j) = A(i, j) - l * A(k, j);
+      r[i] = r[i] - l * r[k];
+    }
+  }
+  for (int i = nt - 1; i >= 0; --i) {
+    xreal s = r[i];
+    for (int j = i + 1; j < nt; ++j) s = s - A(i, j) * r[j];
+    r[i] = s / A(i, i);
+  }
+  for (int i = 0; i < nt; ++i) out[i] = (double)r[i];
+  return true;
 }
 
 static inline int fdiv(int a, int b) {  // Fortran integer division
@@ -165,11 +204,12 @@ bool der_coeffs_generic(const double *h0, int cl, int der, int so, double imp[5]
     i = (5 - n2) / 2;
     for (int m = 0; m < n2; ++m) h[n1 + m] = h0[i + m];
   }
-  double a[100], r[10] = {0};
+  xreal a[100], r[10] = {0};
+  double sol[10];
   taylor_matrix(h, n1, nt, der, a);
-  r[der] = 1.0;
-  if (!dense_solve(nt, a, r)) return false;
-  export_coeffs(r, cl, n1, n2, cl == 3 ? (5 - n2) / 2 : 0, imp, exp_, 7);
+  r[der] = 1;
+  if (!taylor_solve(nt, a, r, sol)) return false;
+  export_coeffs(sol, cl, n1, n2, cl == 3 ? (5 - n2) / 2 : 0, imp, exp_, 7);
   return true;
 }
 
@@ -295,22 +335,23 @@ bool solver_coeffs_generic(const double *h0, int cl, int der, int bct, int so,
     i = (5 - n2) / 2;
     for (int m = 0; m < n2; ++m) h[n1 + m] = h0[i + m];
   }
-  double a[100], r[10] = {0};
+  xreal a[100], r[10] = {0};
+  double sol[10];
   taylor_matrix(h, n1, n3, der, a);
   // boundary column (:196-203)
   if ((cl == 2 && n2 > 4) || cl == 1) {
     const int c = n1;
-    for (int i = 0; i < bct - 1; ++i) a[i + (size_t)n3 * c] = 0.0;
-    a[bct - 1 + (size_t)n3 * c] = (bct == 3) ? -1.0 : 1.0;
+    for (int i = 0; i < bct - 1; ++i) a[i + (size_t)n3 * c] = 0;
+    a[bct - 1 + (size_t)n3 * c] = (bct == 3) ? -1 : 1;
     for (int i = bct; i < n3; ++i)
-      a[i + (size_t)n3 * c] = a[i - 1 + (size_t)n3 * c] * h[c] / (double)(i + 1 - bct);
+      a[i + (size_t)n3 * c] = a[i - 1 + (size_t)n3 * c] * (xreal)h[c] / (xreal)(i + 1 - bct);
   }
-  r[der] = 1.0;
-  if (!dense_solve(n3, a, r)) return false;
+  r[der] = 1;
+  if (!taylor_solve(n3, a, r, sol)) return false;
   int s = 0;
   if (cl == 3) s = fdiv(5 - n2, 2);
   if (cl == 2) s = std::max(0, fdiv(5 - n2, 2));
-  export_coeffs(r, cl, n1, n2, s, imp, exp_, 8);
+  export_coeffs(sol, cl, n1, n2, s, imp, exp_, 8);
   return true;
 }
 

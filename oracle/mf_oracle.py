@@ -103,20 +103,60 @@ _DER_STENCIL = {
 }
 
 
+# How the small Taylor systems are solved.  "extended": built and solved in
+# x87 80-bit extended precision (numpy.longdouble) by Gaussian elimination
+# with partial pivoting, then rounded to FP64 -- the FP64 LU of the reference
+# (dgesv) spreads by up to ~5e-11 relative across LAPACK builds for the
+# 8th-order second derivative, more than the 1e-12 parity bar, while the
+# extended solve is accurate to ~1e-15.  "lapack": the reference's FP64 dgesv.
+TAYLOR = "extended"
+_XR = np.longdouble
+
+
 def _taylor_matrix(h: np.ndarray, n1: int, nt: int, der: int) -> np.ndarray:
     """Taylor matrix a(nt,nt) (class_derivatives_coefficient.f90:135-151)."""
-    a = np.zeros((nt, nt))
+    dt = _XR if TAYLOR == "extended" else np.float64
+    a = np.zeros((nt, nt), dtype=dt)
+    hh = np.asarray(h, dtype=np.float64).astype(dt)
     # implicit columns: a(der+1,j)=-1, a(i,j)=a(i-1,j)*h(j)/(i-der-1)
-    a[der, :n1] = -1.0
+    a[der, :n1] = -1
     for i in range(der + 1, nt):  # Fortran i = der+2..nt
         for j in range(n1):
-            a[i, j] = a[i - 1, j] * h[j] / (i - der)
+            a[i, j] = a[i - 1, j] * hh[j] / dt(i - der)
     # explicit columns: a(1,j)=1, a(i,j)=a(i-1,j)*h(j)/(i-1)
-    a[0, n1:nt] = 1.0
+    a[0, n1:nt] = 1
     for i in range(1, nt):
         for j in range(n1, nt):
-            a[i, j] = a[i - 1, j] * h[j] / i
+            a[i, j] = a[i - 1, j] * hh[j] / dt(i)
     return a
+
+
+def _taylor_solve(a: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """Solve the Taylor system (dgesv at class_derivatives_coefficient.f90:159,
+    class_solver_coefficient_3d.f90:212); see TAYLOR."""
+    if TAYLOR != "extended":
+        return np.linalg.solve(np.asarray(a, np.float64), np.asarray(r, np.float64))
+    a = np.array(a, dtype=_XR)
+    r = np.array(r, dtype=_XR)
+    nt = a.shape[0]
+    for k in range(nt):
+        col = np.abs(a[k:, k])
+        p = k + int(np.argmax(col))  # first maximal pivot
+        if col.max() == 0:
+            raise np.linalg.LinAlgError("singular Taylor system")
+        if p != k:
+            a[[k, p]] = a[[p, k]]
+            r[[k, p]] = r[[p, k]]
+        for i in range(k + 1, nt):
+            l = a[i, k] / a[k, k]
+            a[i, k:] = a[i, k:] - l * a[k, k:]
+            r[i] = r[i] - l * r[k]
+    for i in range(nt - 1, -1, -1):
+        s = r[i]
+        for j in range(i + 1, nt):
+            s = s - a[i, j] * r[j]
+        r[i] = s / a[i, i]
+    return r.astype(np.float64)
 
 
 def der_coeffs_generic(h0: np.ndarray, cl: int, der: int, so: int):
@@ -147,7 +187,7 @@ def der_coeffs_generic(h0: np.ndarray, cl: int, der: int, so: int):
     a = _taylor_matrix(h, n1, nt, der)
     r = np.zeros(nt)
     r[der] = 1.0
-    r = np.linalg.solve(a, r)  # dgesv, :159
+    r = _taylor_solve(a, r)  # dgesv, :159
     coef_imp = np.zeros(5)
     coef_exp = np.zeros(7)
     # export :167-182
@@ -263,20 +303,30 @@ def pentares(lu: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 
 def derx_rhs(dr: np.ndarray, f: np.ndarray) -> np.ndarray:
-    """Explicit stencil RHS (class_derivatives.f90:262-295); f is (n,) or (n, L)."""
+    """Explicit stencil RHS (class_derivatives.f90:262-295); f is (n,) or (n, L).
+
+    Sums run left to right over the stencil like the reference's explicit
+    expressions (:277-293).
+    """
     n = dr.shape[0]
-    f = np.asarray(f)
+    f = np.asarray(f, dtype=np.float64)
     ext = (slice(None),) + (None,) * (f.ndim - 1)
-    rhs = np.empty_like(f, dtype=np.float64)
-    rhs[0] = np.tensordot(dr[0], f[0:7], axes=(0, 0))
-    rhs[1] = np.tensordot(dr[1], f[0:7], axes=(0, 0))
+    rhs = np.empty_like(f)
+    for row in (0, 1):
+        acc = dr[row, 0] * f[0]
+        for m in range(1, 7):
+            acc = acc + dr[row, m] * f[m]
+        rhs[row] = acc
     if n > 4:
-        mid = np.zeros_like(f[2:n - 2], dtype=np.float64)
-        for m in range(5):
+        mid = dr[2:n - 2, 0][ext] * f[0:n - 4]
+        for m in range(1, 5):
             mid = mid + dr[2:n - 2, m][ext] * f[m:n - 4 + m]
         rhs[2:n - 2] = mid
-    rhs[n - 2] = np.tensordot(dr[n - 2], f[n - 7:n], axes=(0, 0))
-    rhs[n - 1] = np.tensordot(dr[n - 1], f[n - 7:n], axes=(0, 0))
+    for row in (n - 2, n - 1):
+        acc = dr[row, 0] * f[n - 7]
+        for m in range(1, 7):
+            acc = acc + dr[row, m] * f[n - 7 + m]
+        rhs[row] = acc
     return rhs
 
 
@@ -422,7 +472,7 @@ def solver_coeffs_generic(h0: np.ndarray, cl: int, der: int, bct: int, so: int):
             a[i, c] = a[i - 1, c] * h[c] / (i + 1 - bct)
     r = np.zeros(n3)
     r[der] = 1.0
-    r = np.linalg.solve(a, r)  # dgesv, :212
+    r = _taylor_solve(a, r)  # dgesv, :212
     coef_imp = np.zeros(5)
     coef_exp = np.zeros(8)
     j = 0

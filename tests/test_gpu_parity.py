@@ -65,7 +65,11 @@ def test_derivatives_shapes(shape, so):
     rng = np.random.default_rng(sum(shape) + so)
     gs = [o.grid_uniform(shape[0], 0, 1), o.grid_tanh(shape[1], 0, 2, 1.8),
           o.grid_asin(shape[2], -0.5, 0.5)]
-    f = np.asfortranarray(rng.standard_normal(shape))
+    # smooth field + 1e-3 noise, as the reference's derivative tests
+    # (test_derivatives_omp.f90:65-99) and the 64^3 configuration
+    X, Y, Z = np.meshgrid(*gs, indexing="ij")
+    f = np.asfortranarray(np.sin(2 * np.pi * X) + np.cos(np.pi * Y) * np.sin(4 * Z)
+                          + 1e-3 * rng.standard_normal(shape))
     fd = mf.to_device(f)
     for axis in range(3):
         for solver in (False, True):

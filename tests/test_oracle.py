@@ -30,6 +30,64 @@ def test_splitmix64_known_answer():
     assert [int(v) for v in o.splitmix64(1234567, 5)] == ref
 
 
+@pytest.mark.parametrize("so,der,cl", [(8, 2, 3), (8, 2, 1), (6, 2, 2), (6, 1, 3), (4, 2, 1)])
+def test_taylor_solve_extended_vs_mpmath(so, der, cl):
+    """The extended-precision Taylor solve equals a 40-digit mpmath solve of
+    the same FP64 offsets to ~1e-15, and the FP64 dgesv variant is within
+    the system's condition number of it."""
+    import mpmath
+
+    g = o.grid_tanh(41, 0.0, 1.0, 1.6)
+    i = {1: 0, 2: 1, 3: 20}[cl]
+    h0 = np.zeros(7)
+    if cl == 3:
+        h0[0:5] = g[i] - g[i - 2:i + 3]
+    else:
+        h0[:] = g[i] - g[0:7]
+    ci, ce = o.der_coeffs_generic(h0, cl, der, so)
+    n1, n2 = o._DER_STENCIL[cl][so]
+    nt = n1 + n2
+    o.TAYLOR = "lapack"
+    try:
+        ci64, ce64 = o.der_coeffs_generic(h0, cl, der, so)
+    finally:
+        o.TAYLOR = "extended"
+    # the same system in 40-digit arithmetic (offsets selected as the
+    # reference does, class_derivatives_coefficient.f90:59-127)
+    mpmath.mp.dps = 40
+    h = np.zeros(10)
+    if cl == 1:
+        h[0:n1] = h0[1:n1 + 1]; h[n1:nt] = h0[0:n2]
+    elif cl == 2:
+        h[0] = h0[0]; h[1:n1] = h0[2:n1 + 1]; h[n1:nt] = h0[0:n2]
+    else:
+        s = (4 - n1) // 2
+        h[0:2] = h0[s:2 + s]
+        if n1 >= 3 - s:
+            h[2 - s:n1] = h0[3:n1 + 1 + s]
+        s = (5 - n2) // 2
+        h[n1:nt] = h0[s:n2 + s]
+    A = mpmath.zeros(nt, nt)
+    for j in range(n1):
+        A[der, j] = -1
+        for r in range(der + 1, nt):
+            A[r, j] = A[r - 1, j] * mpmath.mpf(h[j]) / (r - der)
+    for j in range(n1, nt):
+        A[0, j] = 1
+        for r in range(1, nt):
+            A[r, j] = A[r - 1, j] * mpmath.mpf(h[j]) / r
+    rhs = mpmath.zeros(nt, 1)
+    rhs[der] = 1
+    x = mpmath.lu_solve(A, rhs)
+    exact = np.array([float(x[m]) for m in range(nt)])
+    s = (5 - n2) // 2 if cl == 3 else 0
+    scale = np.abs(exact).max()
+    np.testing.assert_allclose(ce[s:s + n2], exact[n1:nt], atol=4e-15 * scale, rtol=0)
+    # FP64 dgesv differs by at most the conditioning of the system
+    cond = float(mpmath.norm(A, 1) * mpmath.norm(mpmath.inverse(A), 1))
+    assert np.abs(ce64 - ce).max() <= 10 * cond * 1.1e-16 * scale
+
+
 def test_normal_moments():
     z = o.normal(0, 200001)
     assert abs(z.mean()) < 1e-2 and abs(z.std() - 1.0) < 1e-2
