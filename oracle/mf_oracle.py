@@ -286,7 +286,7 @@ def pentares(lu: np.ndarray, b: np.ndarray) -> np.ndarray:
     p, f, c, d, e = (lu[:, m] for m in range(5))
     alpha = lu[0, 0]
     beta = lu[n - 1, 4]
-    x = np.array(b, dtype=np.float64, copy=True)
+    x = np.array(b, dtype=np.result_type(b, np.float64), copy=True)
     x[0] = b[0]
     x[1] = b[1] - x[0] * f[1]
     for i in range(2, n - 1):
@@ -309,7 +309,8 @@ def derx_rhs(dr: np.ndarray, f: np.ndarray) -> np.ndarray:
     expressions (:277-293).
     """
     n = dr.shape[0]
-    f = np.asarray(f, dtype=np.float64)
+    f = np.asarray(f)
+    f = f.astype(np.result_type(f, np.float64), copy=False)
     ext = (slice(None),) + (None,) * (f.ndim - 1)
     rhs = np.empty_like(f)
     for row in (0, 1):
@@ -772,6 +773,91 @@ def solver_3d(sc: SolverCoeffs3D, g, u, bcx, bcy, bcz, sigma: float):
     """Field wrapper solver_3d (class_field.f90:1072-1109): g, u full-size."""
     gr = np.asfortranarray(g[1:-1, 1:-1, 1:-1])
     return solve_3d(sc, gr, sigma, u, bcx, bcy, bcz)
+
+
+# ----------------------------------------------------------------------------
+# Exact evaluation helpers (tests only).  Two FP64 implementations of the same
+# algorithm differ by their rounding, which for strongly cancelling second
+# derivatives and for the diagonalisation solve reaches a few 1e-12; parity is
+# therefore measured against the algorithm evaluated in extended precision.
+# ----------------------------------------------------------------------------
+
+
+def derivative_3d_extended(dc: DerCoeffs, x: np.ndarray, axis: int, order: int) -> np.ndarray:
+    """derivative_3d with the same FP64 tables, evaluated in 80-bit arithmetic."""
+    return derivative_3d(dc, np.asarray(x, dtype=np.longdouble), axis, order)
+
+
+def _band_apply(mat: np.ndarray, u: np.ndarray, axis: int) -> np.ndarray:
+    """mat (n x n, banded) applied along axis, in u's dtype, diagonal by diagonal."""
+    n = mat.shape[0]
+    ul = np.moveaxis(u, axis, 0)
+    out = np.zeros_like(ul)
+    m = np.asarray(mat, dtype=u.dtype)
+    for off in range(-(n - 1), n):
+        d = np.diagonal(m, off)
+        if not np.any(d):
+            continue
+        ext = (slice(None),) + (None,) * (ul.ndim - 1)
+        if off >= 0:
+            out[0:n - off] += d[ext] * ul[off:n]
+        else:
+            out[-off:n] += d[ext] * ul[0:n + off]
+    return np.moveaxis(out, 0, axis)
+
+
+def multiplied_residual(sc: "SolverCoeffs3D", u_int, g_int, sigma, bcx, bcy, bcz, dtype=np.longdouble):
+    """P (g - L u) with P = lhs_x lhs_y lhs_z, from banded products only.
+
+    P L u = sum_a (prod_{b != a} lhs_b)(rhs_a u + bcol_a bc_a) + sigma P u, the
+    operator of class_solver_3d.f90:104-225 multiplied through by P.
+    """
+    u = np.asarray(u_int, dtype=dtype)
+    g = np.asarray(g_int, dtype=dtype)
+    cs = (sc.cx, sc.cy, sc.cz)
+    bcs = (bcx, bcy, bcz)
+
+    def P(v, skip=None):
+        for a in range(3):
+            if a != skip:
+                v = _band_apply(cs[a].lhs, v, a)
+        return v
+
+    res = P(g - sigma * u)
+    for a in range(3):
+        c = cs[a]
+        n = c.n
+        t = _band_apply(c.rhs, u, a)
+        tl = np.moveaxis(t, a, 0)
+        b = np.asarray(bcs[a], dtype=dtype)
+        tl[0] += c.ddr[0, 0] * b[:, :, 0]
+        tl[1] += c.ddr[1, 0] * b[:, :, 0]
+        tl[n - 2] += c.ddr[n - 2, 7] * b[:, :, 1]
+        tl[n - 1] += c.ddr[n - 1, 6] * b[:, :, 1]
+        res = res - P(t, skip=a)
+    return res
+
+
+def solve_exact(sc: "SolverCoeffs3D", g, bcx, bcy, bcz, sigma, iters: int = 3):
+    """Interior solution of the discrete problem to ~1e-15: the FP64 direct
+    solve refined with residuals taken in 80-bit arithmetic (the correction
+    solves P L d = r' by the FP64 diagonalisation on P^-1 r')."""
+    shape = g.shape
+    u = np.zeros(shape, order="F")
+    solver_3d(sc, g, u, bcx, bcy, bcz, sigma)
+    ui = np.array(u[1:-1, 1:-1, 1:-1], dtype=np.longdouble)
+    gi = g[1:-1, 1:-1, 1:-1]
+    zero = [np.zeros_like(b) for b in (bcx, bcy, bcz)]
+    for _ in range(iters):
+        rp = multiplied_residual(sc, ui, gi, sigma, bcx, bcy, bcz)
+        # r = P^-1 r' (FP64 line solves with the dense lhs)
+        r = np.asarray(rp, dtype=np.float64)
+        for a, c in enumerate((sc.cx, sc.cy, sc.cz)):
+            r = np.moveaxis(np.tensordot(c.inv_lhs, np.moveaxis(r, a, 0), axes=(1, 0)), 0, a)
+        d = np.zeros(shape, order="F")
+        solve_3d(sc, np.asfortranarray(r), sigma, d, *zero)
+        ui = ui + d[1:-1, 1:-1, 1:-1].astype(np.longdouble)
+    return ui
 
 
 # ----------------------------------------------------------------------------
