@@ -384,7 +384,8 @@ def derivative_3d(dc: DerCoeffs, x: np.ndarray, axis: int, order: int) -> np.nda
     coefficients (:648-671).  Points the reference leaves undefined (line ends
     of boundary-plane lines, and the 4 never-written edge lines) are 0 here.
     """
-    x = np.asarray(x, dtype=np.float64)
+    x = np.asarray(x)
+    x = x.astype(np.result_type(x, np.float64), copy=False)
     xl = np.moveaxis(x, axis, 0)  # (n, a, b)
     n, na, nb = xl.shape
     out = np.zeros_like(xl)
