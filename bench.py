@@ -1,0 +1,196 @@
+"""Benchmark: 256^3 Helmholtz solves/s, mixed precision vs FP64, on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+
+Workload: single-domain 256^3 box [0,1]^3, tanh stretch 2.0 in y, uniform x
+and z, homogeneous Dirichlet walls; RHS = 8 seeded random-phase cos modes
+(wavenumbers 1-16) + 1e-3 white noise.  sigma is the prediction step's
+Helmholtz coefficient -Re*fac(1) (BDF2, Re=3000, dt=1e-3).  A step is one
+complete solve g -> u with inputs resident in HBM.  The mixed solve (FP32 MFMA
+transforms, FP64-residual refinement) iterates to a relative residual
+<= 1e-12; the FP64 solve is the reference algorithm in FP64 on MFMA.
+
+Prints one JSON line (value = mixed solves/s) with the roofline of the
+dominant kernel (FP32 transform GEMM; average launch time from HIP events
+over the timed region) and the CPU baseline (the FP64 oracle on the host's
+cores, a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mflops3d-md_amd"))
+
+METRIC = "Helmholtz solves/s @256³ mixed vs FP64 at ≤1e-12 rel residual; NS steps/s"
+PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-12)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-solves", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import mflops as mf
+    from mflops import synthetic
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    n = args.n
+    re, dt = 3000.0, 1e-3
+    sigma = -re * 1.5 / dt
+    log("building %d^3 problem" % n)
+    gx, gy, gz, g = synthetic.helmholtz_256_problem(n)
+    sc = mf.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
+    gd = mf.to_device(g)
+    bc = mf.BoundaryCondition.zeros(n, n, n)
+    u64 = mf.fortran_zeros((n, n, n))
+    umx = mf.fortran_zeros((n, n, n))
+
+    def run(prec, u, k):
+        rep = None
+        for _ in range(k):
+            rep = sc.solve(gd, u, bc, sigma, prec, tol=args.tol)
+        return rep
+
+    log("warmup")
+    run(mf.MF_FP64, u64, args.warmup)
+    run(mf.MF_MIXED, umx, args.warmup)
+    torch.cuda.synchronize()
+
+    # FP64 timed region
+    sc.profile(True)
+    sc.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(mf.MF_FP64, u64, args.steps)
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / args.steps
+    prof64 = sc.profile_read(reset=True)
+    log("fp64: %.3f ms/solve" % (t64 * 1e3))
+
+    # mixed timed region
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = run(mf.MF_MIXED, umx, args.steps)
+    torch.cuda.synchronize()
+    tmx = (time.perf_counter() - t0) / args.steps
+    profmx = sc.profile_read(reset=True)
+    sc.profile(False)
+    log("mixed: %.3f ms/solve, %d FP32 solves, residual %.2e" % (tmx * 1e3, rep.iterations, rep.residual))
+
+    a64 = mf.to_host(u64)
+    amx = mf.to_host(umx)
+    inner = (slice(1, -1),) * 3
+    scale = float(np.abs(a64[inner]).max())
+    err_mx_64 = float(np.abs(amx[inner] - a64[inner]).max() / scale)
+
+    def roof(p, key, peak):
+        ms, cnt, work = p[key]
+        if cnt == 0:
+            return None
+        avg = ms / cnt * 1e-3
+        ach = work / cnt / avg / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None,
+                "launches": cnt, "avg_launch_ms": round(ms / cnt, 4),
+                "flops_per_launch": work / cnt}
+
+    r32 = roof(profmx, "transform_fp32", PEAK["fp32_mfma_tflops"])
+    r64 = roof(prof64, "transform_fp64", PEAK["fp64_mfma_tflops"])
+    ms_r, cnt_r, work_r = profmx["residual_fp64"]
+    res_gbs = (work_r / cnt_r) / (ms_r / cnt_r * 1e-3) / 1e9 if cnt_r else None
+    breakdown = {k: {"ms_per_solve": round(v[0] / args.steps, 4), "launches_per_solve": v[1] / args.steps}
+                 for k, v in profmx.items() if v[1]}
+
+    out = {
+        "metric": METRIC,
+        "value": round(1.0 / tmx, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tmx * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 MFMA transforms + f64 residual",
+        "data": "synthetic",
+        "config": {"workload": "helmholtz-%d^3-tanh2.0-dirichlet" % n, "n": n,
+                   "interior": n - 2, "sigma": sigma, "tol": args.tol, "so": 6,
+                   "rhs": "8 seeded cos modes k=1..16 + 1e-3 noise (seed 2)"},
+        "roofline": r32,
+        "fp64": {"value": round(1.0 / t64, 3), "unit": "solves/s", "ms_per_step": round(t64 * 1e3, 4),
+                 "roofline": r64},
+        "mixed_over_fp64": round(t64 / tmx, 3),
+        "refinement": {"fp32_solves": rep.iterations, "residual": rep.residual,
+                       "residual_after_first": rep.residual0},
+        "err_mixed_vs_fp64_gpu": err_mx_64,
+        "residual_kernel_gbs": round(res_gbs, 1) if res_gbs else None,
+        "mixed_breakdown": breakdown,
+        "ns_steps_per_s": None,
+    }
+
+    if not args.no_cpu:
+        out["cpu_baseline"], errs = cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx)
+        out.update(errs)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx):
+    """The FP64 oracle (numpy + OpenBLAS dgemm, the reference's algorithm)
+    timed on the host on a bounded sample: cpu_solves complete solves of the
+    same problem.  Also states the GPU results' distance to it."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mf_oracle as o
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # noqa: BLE001
+        cores = os.cpu_count() or 1
+    n = g.shape[0]
+    log("cpu baseline: oracle setup")
+    sc = o.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
+    z = [np.zeros((n - 2, n - 2, 2), order="F") for _ in range(3)]
+    u = np.zeros_like(g)
+    t0 = time.perf_counter()
+    for _ in range(args.cpu_solves):
+        o.solver_3d(sc, g, u, *z, sigma)
+    t = (time.perf_counter() - t0) / args.cpu_solves
+    log("cpu baseline: %.3f s/solve on %d threads" % (t, cores))
+    inner = (slice(1, -1),) * 3
+    scale = float(np.abs(u[inner]).max())
+    errs = {"err_vs_oracle64": {"fp64_gpu": float(np.abs(a64[inner] - u[inner]).max() / scale),
+                                "mixed_gpu": float(np.abs(amx[inner] - u[inner]).max() / scale)}}
+    base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": "%d full %d^3 FP64 solves (numpy/OpenBLAS dgemm)" % (args.cpu_solves, n)}
+    return base, errs
+
+
+if __name__ == "__main__":
+    main()

@@ -127,6 +127,21 @@ int mf_solver3d_solve_host(mf_solver3d *sc, const double *g, double sigma, doubl
 /* Interior sizes of a solver handle (n1 = nx-2, ...). */
 int mf_solver3d_dims(const mf_solver3d *sc, int *n1, int *n2, int *n3);
 
+/* Kernel timing (no counterpart in the reference; its solve_3d has
+ * commented-out cpu_time probes, class_solver_3d.f90:106-259).  While
+ * enabled, every launch of the solve is bracketed by HIP events on its
+ * stream.  mf_solver3d_profile_read synchronises and returns, per kernel
+ * class, the summed milliseconds, launch counts and algorithmic work since
+ * the last reset (flops for the transforms, minimum HBM bytes for the
+ * memory-bound kernels):
+ * classes 0 BC fold, 1 FP64 transform, 2 FP32 transform, 3 FP64 residual,
+ * 4 right-hand side multiply-through, 5 faces. */
+#define MF_PROF_CLASSES 6
+int mf_solver3d_profile(mf_solver3d *sc, int enable);
+int mf_solver3d_profile_read(mf_solver3d *sc, double ms[MF_PROF_CLASSES],
+                             int count[MF_PROF_CLASSES], double work[MF_PROF_CLASSES],
+                             int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -219,6 +219,18 @@ struct mf_solver3d {
   float *F1 = nullptr, *F2 = nullptr;   // interior FP32 work (mixed)
   double *norms = nullptr;              // device scalars
   double *hnorms = nullptr;             // pinned host mirror
+  // kernel timing (mf_solver3d_profile): event pairs recorded on the launch
+  // stream, resolved at read time
+  bool prof = false;
+  struct Mark {
+    int cls;
+    double work;
+  };
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<Mark> marks;
+  double prof_ms[MF_PROF_CLASSES] = {0};
+  double prof_work[MF_PROF_CLASSES] = {0};
+  int prof_cnt[MF_PROF_CLASSES] = {0};
 };
 
 namespace {
@@ -228,6 +240,29 @@ namespace {
     if (e_ != hipSuccess)                                                             \
       return mf::fail(MF_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+enum { P_FOLD = 0, P_T64 = 1, P_T32 = 2, P_RES = 3, P_MULT = 4, P_FACES = 5 };
+
+// Brackets the launches made during its lifetime with a pair of events.
+struct ProfScope {
+  mf_solver3d *h;
+  hipStream_t st;
+  bool on;
+  ProfScope(mf_solver3d *h_, int cls, double work, hipStream_t st_) : h(h_), st(st_), on(h_->prof) {
+    if (!on) return;
+    const size_t k = 2 * h->marks.size();
+    while (h->ev_pool.size() < k + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+      h->ev_pool.push_back(e);
+    }
+    h->marks.push_back({cls, work});
+    (void)hipEventRecord(h->ev_pool[k], st);
+  }
+  ~ProfScope() {
+    if (on) (void)hipEventRecord(h->ev_pool[2 * h->marks.size() - 1], st);
+  }
+};
 
 void transpose(int n, const std::vector<double> &a, std::vector<double> &t) {
   t.resize((size_t)n * n);
@@ -286,8 +321,38 @@ extern "C" int mf_solver3d_dims(const mf_solver3d *sc, int *n1, int *n2, int *n3
   return MF_OK;
 }
 
+extern "C" int mf_solver3d_profile(mf_solver3d *sc, int enable) {
+  if (!sc) return mf::fail(MF_ERR_ARG, "mf_solver3d_profile: null handle");
+  sc->prof = enable != 0;
+  return MF_OK;
+}
+
+extern "C" int mf_solver3d_profile_read(mf_solver3d *sc, double ms[MF_PROF_CLASSES],
+                                        int count[MF_PROF_CLASSES],
+                                        double work[MF_PROF_CLASSES], int reset) {
+  if (!sc) return mf::fail(MF_ERR_ARG, "mf_solver3d_profile_read: null handle");
+  for (size_t m = 0; m < sc->marks.size(); ++m) {
+    HIPCHK(hipEventSynchronize(sc->ev_pool[2 * m + 1]));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, sc->ev_pool[2 * m], sc->ev_pool[2 * m + 1]));
+    const int c = sc->marks[m].cls;
+    sc->prof_ms[c] += t;
+    sc->prof_work[c] += sc->marks[m].work;
+    sc->prof_cnt[c] += 1;
+  }
+  sc->marks.clear();
+  for (int c = 0; c < MF_PROF_CLASSES; ++c) {
+    if (ms) ms[c] = sc->prof_ms[c];
+    if (count) count[c] = sc->prof_cnt[c];
+    if (work) work[c] = sc->prof_work[c];
+    if (reset) { sc->prof_ms[c] = 0; sc->prof_work[c] = 0; sc->prof_cnt[c] = 0; }
+  }
+  return MF_OK;
+}
+
 extern "C" int mf_solver3d_destroy(mf_solver3d *sc) {
   if (!sc) return MF_OK;
+  for (hipEvent_t e : sc->ev_pool) (void)hipEventDestroy(e);
   if (sc->dmem) (void)hipFree(sc->dmem);
   if (sc->hnorms) (void)hipHostFree(sc->hnorms);
   delete sc;
@@ -454,9 +519,11 @@ namespace {
 using mf::EpiArgs;
 
 template <typename T, typename TA, typename TB, typename TC, int BM, int BN, int EPI>
-void launch_gemm(const TA *A, long lda, long sA, const TB *B, long ldb, TC *C, long ldc, long sC,
-                 int M, int N, int K, int batch, const EpiArgs &ep, hipStream_t st) {
+void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, long ldb, TC *C,
+                 long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
+                 hipStream_t st) {
   dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  ProfScope ps(h, sizeof(T) == 8 ? P_T64 : P_T32, 2.0 * M * N * K * batch, st);
   hipLaunchKernelGGL((mf::gemm_mode_kernel<T, TA, TB, TC, BM, BN, EPI>), grid, dim3(256), 0, st, A,
                      lda, sA, B, ldb, C, ldc, sC, M, N, K, ep);
 }
@@ -480,25 +547,26 @@ void transforms(mf_solver3d *h, const TIn *in, T *w1, T *w2, const T *Wx, const 
   ep.nx = h->nx;
   ep.ny = h->ny;
   // forward X: w1 = Wx . in
-  launch_gemm<T, T, TIn, T, 256, 64, mf::EPI_STORE>(Wx, n1, 0, in, n1, w1, n1, 0, n1, n2 * n3, n1, 1, ep, st);
+  launch_gemm<T, T, TIn, T, 256, 64, mf::EPI_STORE>(h, Wx, n1, 0, in, n1, w1, n1, 0, n1, n2 * n3, n1, 1, ep, st);
   // forward Y: w2_k = w1_k . Wyt
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(w1, n1, n12, Wyt, n2, w2, n1, n12, n1, n2, n2, n3, ep, st);
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(h, w1, n1, n12, Wyt, n2, w2, n1, n12, n1, n2, n2, n3, ep, st);
   // forward Z with the eigenvalue divide: w1 = (w2 . Wzt) / lambda
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_DIVIDE>(w2, n12, 0, Wzt, n3, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_DIVIDE>(h, w2, n12, 0, Wzt, n3, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
   // backward Z: w2 = w1 . Vzt
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(w1, n12, 0, Vzt, n3, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(h, w1, n12, 0, Vzt, n3, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
   // backward Y: w1_k = w2_k . Vyt
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(w2, n1, n12, Vyt, n2, w1, n1, n12, n1, n2, n2, n3, ep, st);
+  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(h, w2, n1, n12, Vyt, n2, w1, n1, n12, n1, n2, n2, n3, ep, st);
   // backward X into u's interior
   if (accumulate)
-    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER_ADD>(Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER_ADD>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
   else
-    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER>(Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
 }
 
 int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy,
                  const double *bcz, hipStream_t st) {
   const long nf = 2 * ((long)h->n2 * h->n3 + (long)h->n1 * h->n3 + (long)h->n1 * h->n2);
+  ProfScope ps(h, P_FACES, 16.0 * nf, st);
   hipLaunchKernelGGL(mf::faces_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, u,
                      h->n1, h->n2, h->n3, h->bctd, h->neum, bcx, bcy, bcz);
   HIPCHK(hipGetLastError());
@@ -519,6 +587,7 @@ int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime
   mf::Band none = {nullptr, nullptr};
   const double *dnull = nullptr;
   double *wnull = nullptr;
+  ProfScope ps(h, P_RES, 24.0 * N, st);  // algorithmic: read u and b', write r'
   hipLaunchKernelGGL((mf::band_kernel<2, 1>), dim3(blocks), dim3(256), 0, st, u0, ldj, ldk, dnull,
                      h->T3, h->T4, n1, n2, n3, h->M[2], h->R[2], none, sigma, dnull, wnull);
   hipLaunchKernelGGL((mf::band_kernel<1, 2>), dim3(blocks), dim3(256), 0, st, (const double *)h->T3,
@@ -540,6 +609,7 @@ int multiply_through(mf_solver3d *h, double *norm2_dev, hipStream_t st) {
   mf::Band none = {nullptr, nullptr};
   const double *dnull = nullptr;
   double *wnull = nullptr;
+  ProfScope ps(h, P_MULT, 16.0 * N, st);  // algorithmic: read g_f, write b'
   hipLaunchKernelGGL((mf::band_kernel<2, 0>), dim3(blocks), dim3(256), 0, st, (const double *)h->T1,
                      il, ik, dnull, h->T3, wnull, n1, n2, n3, h->M[2], none, none, 0.0, dnull, wnull);
   hipLaunchKernelGGL((mf::band_kernel<1, 0>), dim3(blocks), dim3(256), 0, st, (const double *)h->T3,
@@ -567,8 +637,11 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   const unsigned blocks = (unsigned)((N + 255) / 256);
   if (rep) { rep->iterations = 0; rep->residual = 0.0; rep->residual0 = 0.0; }
   if (precision == MF_FP64) {
-    hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T2, h->n1, h->n2,
-                       h->n3, (const double *)h->fold, bcx, bcy, bcz);
+    {
+      ProfScope ps(h, P_FOLD, 16.0 * N, st);
+      hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T2, h->n1, h->n2,
+                         h->n3, (const double *)h->fold, bcx, bcy, bcz);
+    }
     transforms<double, double>(h, h->T2, h->T1, h->T2, h->Wx, h->Wyt, h->Wzt, h->Vx, h->Vyt,
                                h->Vzt, sigma, u, false, st);
     HIPCHK(hipGetLastError());
@@ -577,8 +650,11 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   // ---- mixed precision: FP32 MFMA transforms, FP64 residual refinement
   if (!(tol > 0.0)) tol = 1e-12;
   if (max_iter <= 0) max_iter = 8;
-  hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T1, h->n1, h->n2,
-                     h->n3, (const double *)h->fold, bcx, bcy, bcz);
+  {
+    ProfScope ps(h, P_FOLD, 16.0 * N, st);
+    hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T1, h->n1, h->n2,
+                       h->n3, (const double *)h->fold, bcx, bcy, bcz);
+  }
   HIPCHK(hipMemsetAsync(h->norms, 0, 8 * sizeof(double), st));
   int rc = multiply_through(h, h->norms + 0, st);
   if (rc != MF_OK) return rc;

@@ -267,6 +267,23 @@ class SolverCoeffs3D:
     def handle(self):
         return self._h
 
+    def profile(self, enable: bool = True) -> None:
+        """Bracket every kernel launch of subsequent solves with HIP events."""
+        check(self._lib.mf_solver3d_profile(self._h, int(bool(enable))))
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{class: (milliseconds, launches, algorithmic work)} since the last reset."""
+        from ._lib import PROF_CLASSES
+
+        n = len(PROF_CLASSES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_int * n)()
+        work = (C.c_double * n)()
+        check(self._lib.mf_solver3d_profile_read(self._h, C.cast(ms, C.c_void_p),
+                                                 C.cast(cnt, C.c_void_p),
+                                                 C.cast(work, C.c_void_p), int(bool(reset))))
+        return {k: (ms[i], cnt[i], work[i]) for i, k in enumerate(PROF_CLASSES)}
+
     def solve(self, g, u, bc: BoundaryCondition, sigma: float, precision: int = MF_FP64,
               tol: float = 1e-12, max_iter: int = 8, stream=None) -> SolveReport:
         """solver_3d(sc, g, u, bc, sigma): u's interior and faces are overwritten."""

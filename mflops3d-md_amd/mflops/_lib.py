@@ -38,7 +38,12 @@ SIGNATURES = {
     "mf_solver3d_solve_host": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp, _vp, _vp, C.c_int,
                                          C.c_double, C.c_int, C.POINTER(SolveReport)]),
     "mf_solver3d_dims": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
+    "mf_solver3d_profile_read": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
 }
+
+PROF_CLASSES = ("fold", "transform_fp64", "transform_fp32", "residual_fp64",
+                "multiply_through", "faces")
 
 _lib = None
 

@@ -1,0 +1,76 @@
+"""Grids and seeded synthetic inputs for the benchmark configurations.
+
+Grids follow the reference's mesh generator (src/class_mesh/class_mesh.f90:112-335):
+uniform, the live asin/cos stretching (:181-200) and the tanh stretching
+present there as a comment (:203).  Random numbers come from SplitMix64
+(published constants) -> 53-bit uniforms -> Box-Muller normals, so any
+implementation of the same definition reproduces the inputs bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def grid_uniform(n: int, a: float = 0.0, b: float = 1.0) -> np.ndarray:
+    return a + np.arange(n, dtype=np.float64) * ((b - a) / (n - 1))
+
+
+def grid_tanh(n: int, a: float, b: float, alpha: float) -> np.ndarray:
+    s = np.arange(n, dtype=np.float64) / (n - 1)
+    return a + (b - a) * (1.0 + np.tanh(alpha * (2.0 * s - 1.0)) / np.tanh(alpha)) / 2.0
+
+
+def grid_asin(n: int, a: float, b: float, beta: float = 33.3) -> np.ndarray:
+    alpha = 1.0 - beta / float(n * n)
+    xi = a + np.arange(n, dtype=np.float64) * ((b - a) / (n - 1))
+    return a + (b - a) * (np.arcsin(-alpha * np.cos(np.pi * (xi - a) / (b - a)))
+                          / np.arcsin(alpha) + 1.0) * 0.5
+
+
+def splitmix64(seed: int, count: int, offset: int = 0) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        idx = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, count: int, offset: int = 0) -> np.ndarray:
+    return (splitmix64(seed, count, offset) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal(seed: int, count: int, offset: int = 0) -> np.ndarray:
+    m = (count + 1) // 2
+    u = uniform01(seed, 2 * m, offset)
+    r = np.sqrt(-2.0 * np.log(1.0 - u[0::2]))
+    z = np.empty(2 * m)
+    z[0::2] = r * np.cos(2.0 * np.pi * u[1::2])
+    z[1::2] = r * np.sin(2.0 * np.pi * u[1::2])
+    return z[:count]
+
+
+def helmholtz_256_problem(n: int = 256, stretch: float = 2.0, seed: int = 2, modes: int = 8,
+                          kmax: int = 16, noise: float = 1e-3):
+    """Single-domain n^3 box [0,1]^3, tanh-stretched in y, uniform x and z;
+    RHS = sum of `modes` seeded random-phase cos modes with wavenumbers
+    1..kmax per direction plus `noise` white noise; homogeneous Dirichlet.
+
+    Returns (gx, gy, gz, g) with g a Fortran-ordered (n,n,n) array.
+    """
+    gx = grid_uniform(n, 0.0, 1.0)
+    gy = grid_tanh(n, 0.0, 1.0, stretch)
+    gz = grid_uniform(n, 0.0, 1.0)
+    u = uniform01(seed, 8 * modes)
+    g = np.zeros((n, n, n), order="F")
+    for m in range(modes):
+        k = 1 + np.floor(kmax * u[8 * m:8 * m + 3]).astype(int)
+        ph = 2.0 * np.pi * u[8 * m + 3:8 * m + 6]
+        amp = 0.5 + u[8 * m + 6]
+        fx = np.cos(np.pi * k[0] * gx + ph[0])
+        fy = np.cos(np.pi * k[1] * gy + ph[1])
+        fz = np.cos(np.pi * k[2] * gz + ph[2])
+        g += amp * fx[:, None, None] * fy[None, :, None] * fz[None, None, :]
+    g += noise * normal(seed, n ** 3, offset=8 * modes).reshape((n, n, n), order="F")
+    return gx, gy, gz, np.asfortranarray(g)
