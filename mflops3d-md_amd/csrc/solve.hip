@@ -44,12 +44,12 @@ __global__ void __launch_bounds__(256)
                 const double *__restrict__ fold,  // fx1 fx2 fy1 fy2 fz1 fz2
                 const double *__restrict__ bcx, const double *__restrict__ bcy,
                 const double *__restrict__ bcz) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long N = (long)n1 * n2 * n3;
-  if (p >= N) return;
-  const int i = (int)(p % n1);
-  const long jk = p / n1;
-  const int j = (int)(jk % n2), k = (int)(jk / n2);
+  // launch: block (64,4), grid (ceil(n1/64), ceil(n2/4), n3)
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= n1 || j >= n2) return;
+  const long p = i + (long)n1 * (j + (long)n2 * k);
   const int nx = n1 + 2, ny = n2 + 2;
   const double *fx1 = fold, *fx2 = fx1 + n1, *fy1 = fx2 + n1, *fy2 = fy1 + n2;
   const double *fz1 = fy2 + n2, *fz2 = fz1 + n3;
@@ -126,13 +126,14 @@ __global__ void __launch_bounds__(256)
   // MODE 2: out1 = B1 in ; out2 = B2 in + B1 in2              (y-pass)
   // MODE 3: out1 = bprime - (B2 in + B1 in2), norm             (x-pass)
   // MODE 4: out1 = B1 in, norm of out1                         (b' x-pass)
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long N = (long)n1 * n2 * n3;
+  // launch: block (64,4), grid (ceil(n1/64), ceil(n2/4), n3); the norm is
+  // reduced per block into norm2[block] (deterministic; summed afterwards).
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
   double contrib = 0.0;
-  if (p < N) {
-    const int i = (int)(p % n1);
-    const long jk = p / n1;
-    const int j = (int)(jk % n2), k = (int)(jk / n2);
+  if (i < n1 && j < n2) {
+    const long p = i + (long)n1 * (j + (long)n2 * k);
     const int row = AXIS == 0 ? i : (AXIS == 1 ? j : k);
     const long stride = AXIS == 0 ? 1 : (AXIS == 1 ? ldj : ldk);
     const long inbase = (AXIS == 0 ? 0 : i) + (AXIS == 1 ? 0 : (long)j * ldj) +
@@ -186,10 +187,30 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (MODE == 3 || MODE == 4) {
-    // wavefront reduction then one atomic per wave
+    // wavefront reduction, then the 4 wave sums in fixed order
+    __shared__ double wsum[4];
     for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(norm2, contrib);
+    if (threadIdx.x == 0) wsum[threadIdx.y] = contrib;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      norm2[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
+          (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
   }
+}
+
+// out[0] = sum of n partial sums (one block of 1024 threads, fixed order).
+__global__ void __launch_bounds__(1024) sum_kernel(const double *__restrict__ part, long n,
+                                                   double *__restrict__ out) {
+  __shared__ double s[1024];
+  double a = 0.0;
+  for (long q = threadIdx.x; q < n; q += 1024) a += part[q];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
 }
 
 }  // namespace mf
@@ -218,6 +239,7 @@ struct mf_solver3d {
   double *T3 = nullptr, *T4 = nullptr, *T5 = nullptr;  // residual work (mixed)
   float *F1 = nullptr, *F2 = nullptr;   // interior FP32 work (mixed)
   double *norms = nullptr;              // device scalars
+  double *part = nullptr;               // per-block partial sums of norms
   double *hnorms = nullptr;             // pinned host mirror
   // kernel timing (mf_solver3d_profile): event pairs recorded on the launch
   // stream, resolved at read time
@@ -443,6 +465,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
   // device layout: one allocation carved into 256-byte aligned arrays; the
   // carve runs twice, first to size the allocation, then to assign pointers
   const size_t N = (size_t)n[0] * n[1] * n[2];
+  const size_t nparts = (size_t)((n[0] + 63) / 64) * ((n[1] + 3) / 4) * n[2];
   double *dW[3], *dV[3];
   float *fW[3], *fV[3];
   auto carve = [&](double *base) -> size_t {
@@ -462,6 +485,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->neum = take(neum.size());
     h->bandw = take(bandw.size());
     h->norms = take(8);
+    h->part = take(nparts);
     h->T1 = take(N); h->T2 = take(N); h->T3 = take(N); h->T4 = take(N); h->T5 = take(N);
     h->F1 = (float *)take((N + 1) / 2);
     h->F2 = (float *)take((N + 1) / 2);
@@ -580,7 +604,6 @@ int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime
              double *norm2_dev, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
-  const unsigned blocks = (unsigned)((N + 255) / 256);
   const long ldj = h->nx, ldk = (long)h->nx * h->ny;
   const double *u0 = u + 1 + ldj + ldk;
   const long il = n1, ik = (long)n1 * n2;
@@ -588,14 +611,18 @@ int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime
   const double *dnull = nullptr;
   double *wnull = nullptr;
   ProfScope ps(h, P_RES, 24.0 * N, st);  // algorithmic: read u and b', write r'
-  hipLaunchKernelGGL((mf::band_kernel<2, 1>), dim3(blocks), dim3(256), 0, st, u0, ldj, ldk, dnull,
+  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3), blk(64, 4);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  hipLaunchKernelGGL((mf::band_kernel<2, 1>), grid, blk, 0, st, u0, ldj, ldk, dnull,
                      h->T3, h->T4, n1, n2, n3, h->M[2], h->R[2], none, sigma, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<1, 2>), dim3(blocks), dim3(256), 0, st, (const double *)h->T3,
+  hipLaunchKernelGGL((mf::band_kernel<1, 2>), grid, blk, 0, st, (const double *)h->T3,
                      il, ik, (const double *)h->T4, h->T5, h->T1, n1, n2, n3, h->M[1], h->R[1], none,
                      0.0, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<0, 3>), dim3(blocks), dim3(256), 0, st, (const double *)h->T1,
+  hipLaunchKernelGGL((mf::band_kernel<0, 3>), grid, blk, 0, st, (const double *)h->T1,
                      il, ik, (const double *)h->T5, rp, wnull, n1, n2, n3, h->R[0], h->M[0], none,
-                     0.0, bprime, norm2_dev);
+                     0.0, bprime, h->part);
+  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part, nb,
+                     norm2_dev);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
@@ -604,18 +631,21 @@ int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime
 int multiply_through(mf_solver3d *h, double *norm2_dev, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
-  const unsigned blocks = (unsigned)((N + 255) / 256);
   const long il = n1, ik = (long)n1 * n2;
   mf::Band none = {nullptr, nullptr};
   const double *dnull = nullptr;
   double *wnull = nullptr;
   ProfScope ps(h, P_MULT, 16.0 * N, st);  // algorithmic: read g_f, write b'
-  hipLaunchKernelGGL((mf::band_kernel<2, 0>), dim3(blocks), dim3(256), 0, st, (const double *)h->T1,
+  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3), blk(64, 4);
+  const long nb = (long)grid.x * grid.y * grid.z;
+  hipLaunchKernelGGL((mf::band_kernel<2, 0>), grid, blk, 0, st, (const double *)h->T1,
                      il, ik, dnull, h->T3, wnull, n1, n2, n3, h->M[2], none, none, 0.0, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<1, 0>), dim3(blocks), dim3(256), 0, st, (const double *)h->T3,
+  hipLaunchKernelGGL((mf::band_kernel<1, 0>), grid, blk, 0, st, (const double *)h->T3,
                      il, ik, dnull, h->T4, wnull, n1, n2, n3, h->M[1], none, none, 0.0, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<0, 4>), dim3(blocks), dim3(256), 0, st, (const double *)h->T4,
+  hipLaunchKernelGGL((mf::band_kernel<0, 4>), grid, blk, 0, st, (const double *)h->T4,
                      il, ik, dnull, h->T2, wnull, n1, n2, n3, h->M[0], none, none, 0.0, dnull,
+                     h->part);
+  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part, nb,
                      norm2_dev);
   HIPCHK(hipGetLastError());
   return MF_OK;
@@ -634,12 +664,11 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   if (!std::isfinite(sigma)) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: sigma is not finite");
   hipStream_t st = (hipStream_t)stream;
   const long N = (long)h->n1 * h->n2 * h->n3;
-  const unsigned blocks = (unsigned)((N + 255) / 256);
   if (rep) { rep->iterations = 0; rep->residual = 0.0; rep->residual0 = 0.0; }
   if (precision == MF_FP64) {
     {
       ProfScope ps(h, P_FOLD, 16.0 * N, st);
-      hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T2, h->n1, h->n2,
+      hipLaunchKernelGGL(mf::fold_kernel, dim3((h->n1 + 63) / 64, (h->n2 + 3) / 4, h->n3), dim3(64, 4), 0, st, g, h->T2, h->n1, h->n2,
                          h->n3, (const double *)h->fold, bcx, bcy, bcz);
     }
     transforms<double, double>(h, h->T2, h->T1, h->T2, h->Wx, h->Wyt, h->Wzt, h->Vx, h->Vyt,
@@ -652,7 +681,7 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   if (max_iter <= 0) max_iter = 8;
   {
     ProfScope ps(h, P_FOLD, 16.0 * N, st);
-    hipLaunchKernelGGL(mf::fold_kernel, dim3(blocks), dim3(256), 0, st, g, h->T1, h->n1, h->n2,
+    hipLaunchKernelGGL(mf::fold_kernel, dim3((h->n1 + 63) / 64, (h->n2 + 3) / 4, h->n3), dim3(64, 4), 0, st, g, h->T1, h->n1, h->n2,
                        h->n3, (const double *)h->fold, bcx, bcy, bcz);
   }
   HIPCHK(hipMemsetAsync(h->norms, 0, 8 * sizeof(double), st));
