@@ -53,7 +53,32 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, target)
     if verbose:
         print("built", target)
+    build_fortran(verbose)
     return target
+
+
+FC = os.environ.get("FC", "/opt/rocm/llvm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+
+
+def build_fortran(verbose: bool = True):
+    """ISO_C_BINDING modules + the Fortran driver (when amdflang is present)."""
+    if not os.path.exists(FC):
+        if verbose:
+            print("amdflang not found: Fortran front end not built")
+        return None
+    out = os.path.join(FORTRAN, "_build")
+    os.makedirs(out, exist_ok=True)
+    exe = os.path.join(out, "test_solver_3d_mf")
+    cmd = [FC, "-O2", "-J", out, os.path.join(FORTRAN, "mflops.f90"),
+           os.path.join(FORTRAN, "test_solver_3d_mf.f90"), "-o", exe,
+           "-L", LIB, "-lmflops", "-Wl,-rpath,$ORIGIN/../../lib", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=out)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    if verbose:
+        print("built", exe)
+    return exe
 
 
 if __name__ == "__main__":
