@@ -60,10 +60,10 @@ program test_solver_3d_mf
   errmx = rel_err(um, uex)
   diff = maxval(abs(um(2:nx-1,2:ny-1,2:nz-1) - u(2:nx-1,2:ny-1,2:nz-1))) / &
        maxval(abs(u(2:nx-1,2:ny-1,2:nz-1)))
-  print '(a,es12.4)', 'solver_fp64_error ', err64
-  print '(a,es12.4)', 'solver_mixed_error ', errmx
-  print '(a,es12.4)', 'solver_mixed_vs_fp64 ', diff
-  print '(a,es12.4)', 'solver_mixed_residual ', rep%residual
+  print '(a,es24.16)', 'solver_fp64_error ', err64
+  print '(a,es24.16)', 'solver_mixed_error ', errmx
+  print '(a,es24.16)', 'solver_mixed_vs_fp64 ', diff
+  print '(a,es24.16)', 'solver_mixed_residual ', rep%residual
   call solver_coeffs_destroy_3d(sc)
 
   ! ---- derivatives (test_derivatives_omp.f90, alpha = 1, beta = 2)
@@ -98,7 +98,7 @@ program test_solver_3d_mf
         ! error over points off the box edges (solver-type edges are undefined)
         e = sqrt(sum((df(2:nx-1,2:ny-1,2:nz-1) - dex(2:nx-1,2:ny-1,2:nz-1))**2) / &
              sum(dex(2:nx-1,2:ny-1,2:nz-1)**2))
-        print '(a,i1,a,i1,a,es12.4)', 'derivative_axis', axis, '_order', order, ' ', e
+        print '(a,i1,a,i1,a,es24.16)', 'derivative_axis', axis, '_order', order, ' ', e
         call der_coeffs_destroy(dc)
      end do
   end do
