@@ -127,6 +127,22 @@ int mf_solver3d_solve_host(mf_solver3d *sc, const double *g, double sigma, doubl
 /* Interior sizes of a solver handle (n1 = nx-2, ...). */
 int mf_solver3d_dims(const mf_solver3d *sc, int *n1, int *n2, int *n3);
 
+/* Unit-impulse responses for the influence matrix of a subdomain
+ * (md_influence_matrix_set, src/class_field/class_field.f90:1396-1510).
+ * For impulses first .. first+count-1 on face `face` (0..5 = x1,x2,y1,y2,z1,z2;
+ * face points numbered column-major over the face's interior, as the
+ * reference's loop `do j; do i`), solve with g = 0, a unit Dirichlet value at
+ * that point and zero elsewhere, then write the inward-normal first
+ * derivatives on the six faces (+d/dn on the low face, -d/dn on the high
+ * face, the reference's sign rule :1465-1490) as
+ *   out[b*stride ...] = [bcx(n2,n3,2) | bcy(n1,n3,2) | bcz(n1,n2,2)],
+ * stride = 2(n2 n3 + n1 n3 + n1 n2) (interior sizes).  dx, dy, dz are
+ * solver-mode derivative handles on the subdomain's grids; the impulse face
+ * must be Dirichlet.  out is a device pointer; enqueued on stream. */
+int mf_influence_impulses(mf_solver3d *sc, const mf_der *dx, const mf_der *dy,
+                          const mf_der *dz, double sigma, int face, long long first,
+                          int count, double *out, void *stream);
+
 /* Kernel timing (no counterpart in the reference; its solve_3d has
  * commented-out cpu_time probes, class_solver_3d.f90:106-259).  While
  * enabled, every launch of the solve is bracketed by HIP events on its

@@ -216,6 +216,16 @@ struct mf_der {
   std::vector<double> host[2][2][2];  // [order-1][sub][0: dl_lu(n,5), 1: dr(n,7)]
 };
 
+bool mf::der_host_tables(const mf_der *dc, int order, int sub, const double **dl_lu,
+                         const double **dr, int *n) {
+  if (!dc || order < 1 || order > 2 || (sub && !dc->solver)) return false;
+  const auto &t = dc->host[order - 1][sub ? 1 : 0];
+  *dl_lu = t[0].data();
+  *dr = t[1].data();
+  *n = sub ? dc->n - 2 : dc->n;
+  return true;
+}
+
 namespace {
 // pack one scheme: dr row-major (7n) | p | f | d | e | c
 void pack_scheme(int n, const std::vector<double> &dl, const std::vector<double> &dr,

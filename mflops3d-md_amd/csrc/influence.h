@@ -1,0 +1,135 @@
+// Unit-impulse responses for the influence (capacitance) matrix
+// (md_influence_matrix_set, class_field.f90:1396-1510).
+//
+// For a unit Dirichlet value at one point of face F (g = 0, every other
+// boundary value 0), the BC fold leaves a right-hand side that is nonzero on
+// a single line normal to F: rhs = a (x) e_c1 (x) e_c2 with a = -(fold vector
+// of F).  Its three forward transforms are therefore separable, and the
+// spectral coefficients of a whole batch of impulses are written by one
+// kernel,
+//     S(p,q,r; b) = ahat_p * W_y(q, c1_b) * W_z(r, c2_b) / (lx_p + ly_q + lz_r + sigma)
+// (axes permuted for y- and z-faces), after which the three backward
+// transforms run as batched MFMA GEMMs and face_der_kernel forms the
+// reference's outputs: the inward-normal first derivative on all six faces
+// (+d/dn at the low face, -d/dn at the high face), in the layout of
+// type(boundary_condition) [bcx(n2,n3,2) | bcy(n1,n3,2) | bcz(n1,n2,2)].
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mf {
+
+struct ImpulseArgs {
+  int n1, n2, n3;
+  int face;        // 0..5: x1 x2 y1 y2 z1 z2 (the Dirac's face)
+  long first;      // first impulse index in the face's column-major order
+  const double *ahat;  // W_s (-fold_s) for the source axis s
+  const double *Wx, *Wyt, *Wzt;  // inv_vect_x, inv_vect_y^T, inv_vect_z^T
+  const double *lx, *ly, *lz;
+  double sigma;
+};
+
+// grid: (ceil(n1/64), n2, n3 * B), block 64
+__global__ void __launch_bounds__(64) impulse_spectral_kernel(double *__restrict__ S, ImpulseArgs a) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  const int q = blockIdx.y;
+  const int r = blockIdx.z % a.n3;
+  const int b = blockIdx.z / a.n3;
+  if (p >= a.n1) return;
+  const int s = a.face / 2;
+  const long t = a.first + b;
+  const int L1 = (s == 0) ? a.n2 : a.n1;
+  const int c1 = (int)(t % L1), c2 = (int)(t / L1);
+  double vx, vy, vz;
+  if (s == 0) {
+    vx = a.ahat[p];
+    vy = a.Wyt[c1 + (long)a.n2 * q];
+    vz = a.Wzt[c2 + (long)a.n3 * r];
+  } else if (s == 1) {
+    vx = a.Wx[p + (long)a.n1 * c1];
+    vy = a.ahat[q];
+    vz = a.Wzt[c2 + (long)a.n3 * r];
+  } else {
+    vx = a.Wx[p + (long)a.n1 * c1];
+    vy = a.Wyt[c2 + (long)a.n2 * q];
+    vz = a.ahat[r];
+  }
+  const double lxp = a.lx[p], lyq = a.ly[q], lzr = a.lz[r];
+  const double big = fmax(fmax(fabs(a.sigma), fabs(lxp)), fmax(fabs(lyq), fabs(lzr)));
+  const double v = (big < 1e-11) ? 0.0 : (vx * vy * vz) / (lxp + lyq + lzr + a.sigma);
+  const long N = (long)a.n1 * a.n2 * a.n3;
+  S[b * N + p + (long)a.n1 * (q + (long)a.n2 * r)] = v;
+}
+
+struct FaceDerArgs {
+  int n1, n2, n3;
+  int face;          // source face of the impulses
+  long first;
+  const int *bct;    // 6 boundary codes (device)
+  const double *neum;  // 3 x (2,7) extrapolation weights (device)
+  const double *drow;  // per axis: row0 (n+2) then rown (n+2), axes x, y, z
+  long out_stride;     // doubles per impulse in out
+};
+
+// Lines along axis AX of every batch element: thread per (line, b).
+// grid: (ceil(T1/64), T2, B) with T1 x T2 the transverse interior extents.
+template <int AX>
+__global__ void __launch_bounds__(64) face_der_kernel(const double *__restrict__ U,
+                                                      double *__restrict__ out, FaceDerArgs a) {
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+  const int t1 = blockIdx.x * 64 + threadIdx.x;
+  const int t2 = blockIdx.y;
+  const int b = blockIdx.z;
+  const int T1 = (AX == 0) ? n2 : n1;
+  if (t1 >= T1) return;
+  const int n = (AX == 0) ? n1 : (AX == 1 ? n2 : n3);
+  long base, s;
+  if (AX == 0) { base = (long)n1 * (t1 + (long)n2 * t2); s = 1; }
+  else if (AX == 1) { base = t1 + (long)n1 * n2 * t2; s = n1; }
+  else { base = t1 + (long)n1 * t2; s = (long)n1 * n2; }
+  const long N = (long)n1 * n2 * n3;
+  const double *u = U + (long)b * N + base;
+  // boundary values of the line: Dirichlet -> the impulse (1 at its point
+  // of its face, else 0); Neumann -> extrapolated with zero derivative
+  const long t = a.first + b;
+  const int sa = a.face / 2, side = a.face % 2;
+  const int L1 = (sa == 0) ? n2 : n1;
+  const bool here = (sa == AX) && (t % L1 == t1) && (t / L1 == t2);
+  const double *nm = a.neum + 14 * AX;
+  double bv0, bv1;
+  if (a.bct[2 * AX] == 1) {
+    bv0 = (here && side == 0) ? 1.0 : 0.0;
+  } else {
+    double acc = 0.0;
+    for (int m = 1; m < 7; ++m) acc -= nm[2 * m] * u[(long)(m - 1) * s];
+    bv0 = acc * (1.0 / nm[0]);
+  }
+  if (a.bct[2 * AX + 1] == 1) {
+    bv1 = (here && side == 1) ? 1.0 : 0.0;
+  } else {
+    double acc = 0.0;
+    for (int m = 5; m >= 0; --m) acc -= nm[1 + 2 * m] * u[(long)(n - 6 + m) * s];
+    bv1 = acc * (1.0 / nm[13]);
+  }
+  // rows of the full-line derivative operator (n + 2 points)
+  const int off = (AX == 0) ? 0 : (AX == 1 ? 2 * (n1 + 2) : 2 * (n1 + 2) + 2 * (n2 + 2));
+  const double *r0 = a.drow + off, *rn = r0 + (n + 2);
+  double d0 = r0[0] * bv0, dn = rn[0] * bv0;
+  for (int q = 0; q < n; ++q) {
+    const double v = u[(long)q * s];
+    d0 = fma(r0[q + 1], v, d0);
+    dn = fma(rn[q + 1], v, dn);
+  }
+  d0 = fma(r0[n + 1], bv1, d0);
+  dn = fma(rn[n + 1], bv1, dn);
+  // layout [bcx(n2,n3,2) | bcy(n1,n3,2) | bcz(n1,n2,2)]
+  double *o = out + (long)b * a.out_stride;
+  long fsz, foff;
+  if (AX == 0) { fsz = (long)n2 * n3; foff = 0; }
+  else if (AX == 1) { fsz = (long)n1 * n3; foff = 2L * n2 * n3; }
+  else { fsz = (long)n1 * n2; foff = 2L * n2 * n3 + 2L * n1 * n3; }
+  const long idx = t1 + (long)T1 * t2;
+  o[foff + idx] = d0;
+  o[foff + fsz + idx] = -dn;
+}
+
+}  // namespace mf

@@ -5,6 +5,8 @@
 #include <string>
 #include <vector>
 
+struct mf_der;
+
 namespace mf {
 
 // One direction of the diagonalisation solver (the reference's
@@ -34,6 +36,15 @@ bool extrapolation_coeffs(const double *grid, int n, const int bct[2], int so, d
 bool real_eig(int n, const double *A, double *wr, double *V, double *max_imag, const char **why);
 bool solver_setup_1d(const double *grid, int nall, const int bct[2], int so, Solver1D &out,
                      std::string &err);
+
+// Rows 0 and n-1 of the dense operator D = lhs^-1 rhs of one packed compact
+// scheme (dl_lu after pentalu, dr), i.e. the boundary-point derivative as a
+// dot product with the whole line.
+void derivative_rows(int n, const double *dl_lu, const double *dr, double *row0, double *rown);
+
+// host copies of a derivative handle's tables (order 1|2, sub 0|1)
+bool der_host_tables(const struct ::mf_der *dc, int order, int sub, const double **dl_lu,
+                     const double **dr, int *n);
 
 // thread-local last error message (mf_last_error)
 int fail(int code, const std::string &msg);

@@ -288,6 +288,50 @@ void pentalu(int n, double *dl) {
   c[n - 1] = c[n - 1] - e[n - 3] * p[n - 1] - d[n - 2] * f[n - 1];
 }
 
+// Apply one packed scheme to a line on the host: stencil right-hand side
+// (class_derivatives.f90:262-295) and pentares (:425-466).
+static void apply_scheme(int n, const double *dl, const double *dr, const double *f, double *x) {
+  auto DR = [&](int i, int m) { return dr[i + (size_t)n * m]; };
+  const double *p = dl, *fl = dl + n, *c = dl + 2 * (size_t)n, *d = dl + 3 * (size_t)n,
+               *e = dl + 4 * (size_t)n;
+  const double alpha = p[0], beta = e[n - 1];
+  std::vector<double> b(n);
+  for (int row = 0; row < 2; ++row) {
+    double a = 0.0;
+    for (int m = 0; m < 7; ++m) a += DR(row, m) * f[m];
+    b[row] = a;
+  }
+  for (int i = 2; i < n - 2; ++i) {
+    double a = 0.0;
+    for (int m = 0; m < 5; ++m) a += DR(i, m) * f[i - 2 + m];
+    b[i] = a;
+  }
+  for (int row = n - 2; row < n; ++row) {
+    double a = 0.0;
+    for (int m = 0; m < 7; ++m) a += DR(row, m) * f[n - 7 + m];
+    b[row] = a;
+  }
+  x[0] = b[0];
+  x[1] = b[1] - x[0] * fl[1];
+  for (int i = 2; i < n - 1; ++i) x[i] = b[i] - p[i] * x[i - 2] - fl[i] * x[i - 1];
+  x[n - 1] = b[n - 1] - p[n - 1] * x[n - 3] - fl[n - 1] * x[n - 2] - beta * x[n - 4];
+  x[n - 1] = x[n - 1] / c[n - 1];
+  x[n - 2] = (x[n - 2] - d[n - 2] * x[n - 1]) / c[n - 2];
+  for (int i = n - 3; i >= 1; --i) x[i] = (x[i] - e[i] * x[i + 2] - d[i] * x[i + 1]) / c[i];
+  x[0] = (x[0] - e[0] * x[2] - d[0] * x[1] - alpha * x[3]) / c[0];
+}
+
+void derivative_rows(int n, const double *dl_lu, const double *dr, double *row0, double *rown) {
+  std::vector<double> f(n, 0.0), x(n);
+  for (int m = 0; m < n; ++m) {
+    f[m] = 1.0;
+    apply_scheme(n, dl_lu, dr, f.data(), x.data());
+    row0[m] = x[0];
+    rown[m] = x[n - 1];
+    f[m] = 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Solver coefficients (class_solver_coefficient_3d.f90:30-245)
 // ---------------------------------------------------------------------------

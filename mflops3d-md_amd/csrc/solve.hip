@@ -18,11 +18,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "gemm.h"
+#include "influence.h"
 #include "internal.h"
 #include "mflops/mf.h"
 
@@ -241,6 +243,8 @@ struct mf_solver3d {
   double *norms = nullptr;              // device scalars
   double *part = nullptr;               // per-block partial sums of norms
   double *hnorms = nullptr;             // pinned host mirror
+  void *iws = nullptr;                  // unit-impulse batch workspace (grown on demand)
+  size_t iws_bytes = 0;
   // kernel timing (mf_solver3d_profile): event pairs recorded on the launch
   // stream, resolved at read time
   bool prof = false;
@@ -375,6 +379,7 @@ extern "C" int mf_solver3d_profile_read(mf_solver3d *sc, double ms[MF_PROF_CLASS
 extern "C" int mf_solver3d_destroy(mf_solver3d *sc) {
   if (!sc) return MF_OK;
   for (hipEvent_t e : sc->ev_pool) (void)hipEventDestroy(e);
+  if (sc->iws) (void)hipFree(sc->iws);
   if (sc->dmem) (void)hipFree(sc->dmem);
   if (sc->hnorms) (void)hipHostFree(sc->hnorms);
   delete sc;
@@ -750,4 +755,121 @@ extern "C" int mf_solver3d_solve_host(mf_solver3d *h, const double *g, double si
   if (e != hipSuccess)
     return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_solve_host: ") + hipGetErrorString(e));
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Unit-impulse batch (md_influence_matrix_set, class_field.f90:1396-1510)
+// ---------------------------------------------------------------------------
+extern "C" int mf_influence_impulses(mf_solver3d *h, const mf_der *dx, const mf_der *dy,
+                                     const mf_der *dz, double sigma, int face, long long first,
+                                     int count, double *out, void *stream) {
+  if (!h || !dx || !dy || !dz || !out) return mf::fail(MF_ERR_ARG, "mf_influence_impulses: null pointer");
+  if (face < 0 || face > 5) return mf::fail(MF_ERR_ARG, "mf_influence_impulses: face must be 0..5");
+  if (h->bct[face] != 1)
+    return mf::fail(MF_ERR_ARG, "mf_influence_impulses: the impulse face must be Dirichlet (an interface)");
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  const int s = face / 2;
+  const long fpts = (s == 0) ? (long)n2 * n3 : (s == 1 ? (long)n1 * n3 : (long)n1 * n2);
+  if (count < 0 || first < 0 || first + count > fpts)
+    return mf::fail(MF_ERR_ARG, "mf_influence_impulses: impulse range outside the face");
+  if (count == 0) return MF_OK;
+  if (!std::isfinite(sigma)) return mf::fail(MF_ERR_ARG, "mf_influence_impulses: sigma is not finite");
+  // derivative rows of the three full-line schemes (first derivative)
+  const mf_der *ds[3] = {dx, dy, dz};
+  const int nfull[3] = {h->nx, h->ny, h->nz};
+  std::vector<double> drow;
+  for (int a = 0; a < 3; ++a) {
+    const double *dl = nullptr, *dr = nullptr;
+    int n = 0;
+    if (!mf::der_host_tables(ds[a], 1, 0, &dl, &dr, &n) || n != nfull[a])
+      return mf::fail(MF_ERR_ARG, "mf_influence_impulses: derivative handle does not match the grid");
+    std::vector<double> r0(n), rn(n);
+    mf::derivative_rows(n, dl, dr, r0.data(), rn.data());
+    drow.insert(drow.end(), r0.begin(), r0.end());
+    drow.insert(drow.end(), rn.begin(), rn.end());
+  }
+  // ahat = W_s (-fold of the impulse face)
+  const mf::Solver1D &S1 = h->ax[s];
+  const int ns = S1.n;
+  std::vector<double> fold(ns), ahat(ns, 0.0);
+  {
+    auto IL = [&](int i, int j) { return S1.inv_lhs[i + (size_t)ns * j]; };
+    auto DR = [&](int i, int c) { return S1.ddr[i + (size_t)ns * c]; };
+    for (int i = 0; i < ns; ++i)
+      fold[i] = (face % 2 == 0) ? IL(i, 0) * DR(0, 0) + IL(i, 1) * DR(1, 0)
+                                : IL(i, ns - 2) * DR(ns - 2, 7) + IL(i, ns - 1) * DR(ns - 1, 6);
+    for (int p = 0; p < ns; ++p) {
+      double acc = 0.0;
+      for (int i = 0; i < ns; ++i) acc += S1.inv_vect[p + (size_t)ns * i] * (-fold[i]);
+      ahat[p] = acc;
+    }
+  }
+  // batch size: two interior fields per impulse, within a 2 GB workspace and
+  // the launch-grid limits of the batched transforms
+  const long N = (long)n1 * n2 * n3;
+  long B = (2L << 30) / (2 * N * (long)sizeof(double));
+  B = std::min<long>(B, 65535L / n3);
+  B = std::min<long>(B, (65535L * 64) / ((long)n2 * n3));
+  if (const char *cap = std::getenv("MFLOPS_IMPULSE_BATCH")) {  // smaller batches on request
+    const long c = std::atol(cap);
+    if (c > 0) B = std::min(B, c);
+  }
+  B = std::max<long>(1, std::min<long>(B, count));
+  const size_t small = (size_t)(ns + drow.size() + 64);
+  const size_t need = (2 * (size_t)B * N + small) * sizeof(double);
+  hipStream_t st = (hipStream_t)stream;
+  if (h->iws_bytes < need) {
+    if (h->iws) {
+      HIPCHK(hipStreamSynchronize(st));
+      (void)hipFree(h->iws);
+      h->iws = nullptr;
+      h->iws_bytes = 0;
+    }
+    HIPCHK(hipMalloc(&h->iws, need));
+    h->iws_bytes = need;
+  }
+  double *W1 = (double *)h->iws, *W2 = W1 + (size_t)B * N;
+  double *d_ahat = W2 + (size_t)B * N;
+  double *d_drow = d_ahat + ns;
+  HIPCHK(hipMemcpyAsync(d_ahat, ahat.data(), ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_drow, drow.data(), drow.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope after return
+  const long stride = 2L * n2 * n3 + 2L * n1 * n3 + 2L * n1 * n2;
+  const long n12 = (long)n1 * n2;
+  EpiArgs ep;
+  std::memset(&ep, 0, sizeof(ep));
+  for (long b0 = 0; b0 < count; b0 += B) {
+    const int nb = (int)std::min<long>(B, count - b0);
+    mf::ImpulseArgs ia;
+    ia.n1 = n1; ia.n2 = n2; ia.n3 = n3; ia.face = face; ia.first = first + b0;
+    ia.ahat = d_ahat; ia.Wx = h->Wx; ia.Wyt = h->Wyt; ia.Wzt = h->Wzt;
+    ia.lx = h->lam; ia.ly = h->lam + n1; ia.lz = h->lam + n1 + n2; ia.sigma = sigma;
+    {
+      ProfScope ps(h, P_FOLD, 8.0 * N * nb, st);
+      hipLaunchKernelGGL(mf::impulse_spectral_kernel, dim3((n1 + 63) / 64, n2, n3 * nb), dim3(64),
+                         0, st, W1, ia);
+    }
+    // backward transforms, batched over the impulses
+    launch_gemm<double, double, double, double, 64, 256, mf::EPI_STORE>(
+        h, W1, n12, N, h->Vzt, n3, W2, n12, N, (int)n12, n3, n3, nb, ep, st);
+    launch_gemm<double, double, double, double, 64, 256, mf::EPI_STORE>(
+        h, W2, n1, n12, h->Vyt, n2, W1, n1, n12, n1, n2, n2, n3 * nb, ep, st);
+    launch_gemm<double, double, double, double, 256, 64, mf::EPI_STORE>(
+        h, h->Vx, n1, 0, W1, n1, W2, n1, 0, n1, (int)((long)n2 * n3 * nb), n1, 1, ep, st);
+    mf::FaceDerArgs fa;
+    fa.n1 = n1; fa.n2 = n2; fa.n3 = n3; fa.face = face; fa.first = first + b0;
+    fa.bct = h->bctd; fa.neum = h->neum; fa.drow = d_drow; fa.out_stride = stride;
+    double *o = out + b0 * stride;
+    {
+      ProfScope ps(h, P_FACES, 24.0 * N * nb, st);
+      hipLaunchKernelGGL((mf::face_der_kernel<0>), dim3((n2 + 63) / 64, n3, nb), dim3(64), 0, st,
+                         (const double *)W2, o, fa);
+      hipLaunchKernelGGL((mf::face_der_kernel<1>), dim3((n1 + 63) / 64, n3, nb), dim3(64), 0, st,
+                         (const double *)W2, o, fa);
+      hipLaunchKernelGGL((mf::face_der_kernel<2>), dim3((n1 + 63) / 64, n2, nb), dim3(64), 0, st,
+                         (const double *)W2, o, fa);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  return MF_OK;
 }

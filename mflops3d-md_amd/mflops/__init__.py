@@ -307,3 +307,47 @@ def solver_3d(sc: SolverCoeffs3D, g, u, bc: BoundaryCondition, sigma: float,
               precision: int = MF_FP64, **kw) -> SolveReport:
     """solver_3d (class_field.f90:1072-1109)."""
     return sc.solve(g, u, bc, sigma, precision, **kw)
+
+
+FACES = ("x1", "x2", "y1", "y2", "z1", "z2")
+
+
+def face_points(shape, face: int) -> int:
+    nx, ny, nz = shape
+    n1, n2, n3 = nx - 2, ny - 2, nz - 2
+    return (n2 * n3, n1 * n3, n1 * n2)[face // 2]
+
+
+def influence_impulses(sc: SolverCoeffs3D, dcs, sigma: float, face: int, first: int, count: int,
+                       out=None, stream=None):
+    """Unit-impulse responses of md_influence_matrix_set (class_field.f90:1396-1510).
+
+    Returns (or fills) a device tensor (count, stride) whose row b is
+    [bcx(n2,n3,2) | bcy(n1,n3,2) | bcz(n1,n2,2)] (each Fortran-ordered) of the
+    inward-normal derivatives for impulse first+b on `face`.
+    """
+    torch = _torch()
+    nx, ny, nz = sc.shape
+    n1, n2, n3 = nx - 2, ny - 2, nz - 2
+    stride = 2 * (n2 * n3 + n1 * n3 + n1 * n2)
+    if out is None:
+        out = torch.empty((count, stride), device="cuda", dtype=torch.float64)
+    if out.shape != (count, stride) or not out.is_contiguous():
+        raise ValueError("out must be a contiguous (count, %d) float64 tensor" % stride)
+    dx, dy, dz = dcs
+    check(sc._lib.mf_influence_impulses(sc.handle, dx.handle, dy.handle, dz.handle, float(sigma),
+                                        int(face), int(first), int(count),
+                                        C.c_void_p(out.data_ptr()), _stream(stream)))
+    return out
+
+
+def split_faces(row: np.ndarray, shape):
+    """One output row -> (bcx, bcy, bcz) Fortran arrays."""
+    nx, ny, nz = shape
+    n1, n2, n3 = nx - 2, ny - 2, nz - 2
+    a = 2 * n2 * n3
+    b = a + 2 * n1 * n3
+    bcx = row[:a].reshape((n2, n3, 2), order="F")
+    bcy = row[a:b].reshape((n1, n3, 2), order="F")
+    bcz = row[b:].reshape((n1, n2, 2), order="F")
+    return bcx, bcy, bcz

@@ -38,6 +38,8 @@ SIGNATURES = {
     "mf_solver3d_solve_host": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp, _vp, _vp, C.c_int,
                                          C.c_double, C.c_int, C.POINTER(SolveReport)]),
     "mf_solver3d_dims": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "mf_influence_impulses": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_longlong,
+                                        C.c_int, _vp, _vp]),
     "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
     "mf_solver3d_profile_read": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
 }

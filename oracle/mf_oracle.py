@@ -777,6 +777,73 @@ def solver_3d(sc: SolverCoeffs3D, g, u, bcx, bcy, bcz, sigma: float):
 
 
 # ----------------------------------------------------------------------------
+# Unit-impulse responses (md_influence_matrix_set, class_field.f90:1396-1510)
+# ----------------------------------------------------------------------------
+
+
+def influence_impulses(sc: "SolverCoeffs3D", dcs, sigma: float, face: int, first: int, count: int,
+                       exact: bool = False):
+    """For each impulse on `face` (0..5 = x1,x2,y1,y2,z1,z2; face points in the
+    reference's loop order, i fastest :1431-1432): zero g, u and bc, put a unit
+    value in the face's bc (:1435-1459), solve (:1462), take derx/dery/derz
+    (:1468-1490) and keep the face-normal derivatives with the sign rule
+    (+ at the low face, - at the high face, for m = 1 and m = 2 alike).
+
+    Returns an array (count, 2(n2 n3 + n1 n3 + n1 n2)) of rows
+    [bcx.ravel(F) | bcy.ravel(F) | bcz.ravel(F)].  exact=True evaluates the
+    solve as the refined discrete solution and the derivatives in 80-bit
+    arithmetic (tests only).
+    """
+    nx, ny, nz = sc.cx.nall, sc.cy.nall, sc.cz.nall
+    n1, n2, n3 = nx - 2, ny - 2, nz - 2
+    dcx, dcy, dcz = dcs
+    rows = []
+    for t in range(first, first + count):
+        g = np.zeros((nx, ny, nz), order="F")
+        u = np.zeros((nx, ny, nz), order="F")
+        bcx = np.zeros((n2, n3, 2), order="F")
+        bcy = np.zeros((n1, n3, 2), order="F")
+        bcz = np.zeros((n1, n2, 2), order="F")
+        l, m = face // 2, face % 2
+        if l == 0:
+            bcx[t % n2, t // n2, m] = 1.0
+        elif l == 1:
+            bcy[t % n1, t // n1, m] = 1.0
+        else:
+            bcz[t % n1, t // n1, m] = 1.0
+        solver_3d(sc, g, u, bcx, bcy, bcz, sigma)
+        if exact:
+            ui = solve_exact(sc, g, bcx, bcy, bcz, sigma)
+            u = np.array(u, dtype=np.longdouble)
+            u[1:-1, 1:-1, 1:-1] = ui
+            # faces from the exact interior (Neumann extrapolation)
+            for ax, c in enumerate((sc.cx, sc.cy, sc.cz)):
+                ul = np.moveaxis(u, ax, 0)
+                bc = (bcx, bcy, bcz)[ax]
+                inner = (slice(1, -1), slice(1, -1))
+                if c.bc[0] != 1:
+                    acc = bc[:, :, 0].astype(np.longdouble)
+                    for k in range(1, 7):
+                        acc = acc - c.neum[0, k] * ul[k][inner]
+                    ul[0][inner] = acc * (1.0 / c.neum[0, 0])
+                if c.bc[1] != 1:
+                    nn = ul.shape[0] - 2
+                    acc = bc[:, :, 1].astype(np.longdouble)
+                    for k in range(5, -1, -1):
+                        acc = acc - c.neum[1, k] * ul[nn - 5 + k][inner]
+                    ul[nn + 1][inner] = acc * (1.0 / c.neum[1, 6])
+        dx = derivative_3d(dcx, u, 0, 1)
+        dy = derivative_3d(dcy, u, 1, 1)
+        dz = derivative_3d(dcz, u, 2, 1)
+        ox = np.stack([dx[0, 1:-1, 1:-1], -dx[-1, 1:-1, 1:-1]], -1)
+        oy = np.stack([dy[1:-1, 0, 1:-1], -dy[1:-1, -1, 1:-1]], -1)
+        oz = np.stack([dz[1:-1, 1:-1, 0], -dz[1:-1, 1:-1, -1]], -1)
+        rows.append(np.concatenate([ox.ravel(order="F"), oy.ravel(order="F"),
+                                    oz.ravel(order="F")]).astype(np.float64))
+    return np.array(rows)
+
+
+# ----------------------------------------------------------------------------
 # Exact evaluation helpers (tests only).  Two FP64 implementations of the same
 # algorithm differ by their rounding, which for strongly cancelling second
 # derivatives and for the diagonalisation solve reaches a few 1e-12; parity is

@@ -353,6 +353,56 @@ def test_manufactured_x9_y9_xy():
     assert err < 1e-6, err
 
 
+@pytest.mark.parametrize("face", [0, 3, 5])
+def test_influence_impulses_vs_dense(face):
+    """md_influence_matrix_set restated, against a dense Kronecker solve and
+    dense derivative rows of the same discretisation."""
+    shape = (10, 11, 9)
+    bct = (1, 1, 1, 1, 1, 2) if face != 5 else (1, 2, 1, 1, 1, 1)
+    gs = (o.grid_uniform(10), o.grid_tanh(11, 0, 1, 1.3), o.grid_asin(9, 0, 1))
+    sc = o.SolverCoeffs3D(*gs, bct, 6)
+    dcs = [o.DerCoeffs(g, solver=True, so=6) for g in gs]
+    sigma = -50.0
+    got = o.influence_impulses(sc, dcs, sigma, face, 2, 4)
+    n1, n2, n3 = 8, 9, 7
+    K = _kron_operator(sc, sigma)
+    D = []
+    for g, dc in zip(gs, dcs):
+        n = len(g)
+        E = np.eye(n)
+        D.append(np.stack([dc.der(E[:, m]) for m in range(n)], 1))  # D[:, m] = der(e_m)
+    for b, t in enumerate(range(2, 6)):
+        bcs = [np.zeros((n2, n3, 2)), np.zeros((n1, n3, 2)), np.zeros((n1, n2, 2))]
+        l, m = face // 2, face % 2
+        L1 = n2 if l == 0 else n1
+        bcs[l][t % L1, t // L1, m] = 1.0
+        zero = np.zeros((n1, n2, n3))
+        bpart = o.apply_operator(sc, zero, 0.0, *bcs)
+        ui = np.linalg.solve(K, -bpart.ravel(order="F")).reshape((n1, n2, n3), order="F")
+        u = np.zeros(shape)
+        u[1:-1, 1:-1, 1:-1] = ui
+        for ax, c in enumerate((sc.cx, sc.cy, sc.cz)):   # faces as the solver writes them
+            ul = np.moveaxis(u, ax, 0)
+            inner = (slice(1, -1), slice(1, -1))
+            for side in (0, 1):
+                idx = 0 if side == 0 else ul.shape[0] - 1
+                if c.bc[side] == 1:
+                    ul[idx][inner] = bcs[ax][:, :, side]
+                elif side == 0:
+                    ul[0][inner] = -sum(c.neum[0, k] * ul[k][inner] for k in range(1, 7)) / c.neum[0, 0]
+                else:
+                    nn = ul.shape[0] - 2
+                    ul[nn + 1][inner] = -sum(c.neum[1, k] * ul[nn - 5 + k][inner]
+                                             for k in range(6)) / c.neum[1, 6]
+        dx = np.tensordot(D[0], u, axes=(1, 0))
+        dy = np.moveaxis(np.tensordot(D[1], u, axes=(1, 1)), 0, 1)
+        dz = np.moveaxis(np.tensordot(D[2], u, axes=(1, 2)), 0, 2)
+        ref = np.concatenate([np.stack([dx[0, 1:-1, 1:-1], -dx[-1, 1:-1, 1:-1]], -1).ravel(order="F"),
+                              np.stack([dy[1:-1, 0, 1:-1], -dy[1:-1, -1, 1:-1]], -1).ravel(order="F"),
+                              np.stack([dz[1:-1, 1:-1, 0], -dz[1:-1, 1:-1, -1]], -1).ravel(order="F")])
+        np.testing.assert_allclose(got[b], ref, atol=1e-9 * np.abs(ref).max())
+
+
 def test_poisson_all_neumann_null_mode():
     shape = (10, 11, 9)
     sc = o.SolverCoeffs3D(o.grid_uniform(10), o.grid_tanh(11, 0, 1, 1.2), o.grid_uniform(9), (2,) * 6, 6)
