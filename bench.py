@@ -43,11 +43,14 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-solves", type=int, default=2)
+    ap.add_argument("--workload", choices=("helmholtz", "influence"), default="helmholtz")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "influence":
+        return run_influence(args)
 
     import numpy as np
     import torch
@@ -190,6 +193,81 @@ def cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx):
     base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": cores, "kind": "port",
             "sample": "%d full %d^3 FP64 solves (numpy/OpenBLAS dgemm)" % (args.cpu_solves, n)}
     return base, errs
+
+
+def run_influence(args):
+    """Influence-matrix decomposition on one GPU: 4x4x2 = 32 subdomains with
+    per-slab sizes drawn from {48,64,96} (seed 1), asin-stretched FP64 grids,
+    Dirichlet interfaces; every unit-impulse response of every interface face
+    (md_influence_matrix_set, class_field.f90:1396-1510) in batched launches.
+    Reports local solves/s over the whole set.  Setup (coefficients and
+    eigen-decompositions) is outside the timed region."""
+    import numpy as np
+    import torch
+
+    import mflops as mf
+    from mflops import synthetic
+
+    torch.cuda.set_device(0)
+    sigma = -3000.0 * 1.5 / 1e-3
+    slab, subs = synthetic.subdomain_layout()
+    chunk = 4096
+    total_imp, total_s, total_flop, ref_flop = 0, 0.0, 0.0, 0.0
+    gemm_ms, gemm_work = 0.0, 0.0
+    log("influence: slab sizes x=%s y=%s z=%s" % tuple(slab))
+    for si, sd in enumerate(subs):
+        gx, gy, gz = sd["grids"]
+        shape = (len(gx), len(gy), len(gz))
+        n1, n2, n3 = (s - 2 for s in shape)
+        sc = mf.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
+        dcs = [mf.DerivativeCoefficients(g, solver=True, so=6) for g in (gx, gy, gz)]
+        stride = 2 * (n2 * n3 + n1 * n3 + n1 * n2)
+        out = torch.empty((chunk, stride), device="cuda", dtype=torch.float64)
+        # warm the workspace once (untimed)
+        mf.influence_impulses(sc, dcs, sigma, sd["interfaces"][0], 0, 1, out=out[:1])
+        torch.cuda.synchronize()
+        sc.profile(True)
+        sc.profile_read(reset=True)
+        t0 = time.perf_counter()
+        nimp = 0
+        for face in sd["interfaces"]:
+            npts = mf.face_points(shape, face)
+            for first in range(0, npts, chunk):
+                cnt = min(chunk, npts - first)
+                mf.influence_impulses(sc, dcs, sigma, face, first, cnt, out=out[:cnt])
+                nimp += cnt
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        prof = sc.profile_read(reset=True)
+        sc.profile(False)
+        gemm_ms += prof["transform_fp64"][0]
+        gemm_work += prof["transform_fp64"][2]
+        total_imp += nimp
+        total_s += dt
+        per = 2.0 * n1 * n2 * n3 * (n1 + n2 + n3)
+        total_flop += prof["transform_fp64"][2]
+        ref_flop += nimp * 2 * per  # a full solve per impulse, as the reference does
+        log("subdomain %2d %s: %d impulses in %.3f s" % (si, shape, nimp, dt))
+        del sc, dcs, out
+    achieved = gemm_work / (gemm_ms * 1e-3) / 1e12 if gemm_ms else None
+    out = {
+        "metric": "influence-matrix unit-impulse local solves/s (config 3)",
+        "value": round(total_imp / total_s, 1),
+        "unit": "local solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "influence-4x4x2-{48,64,96}-seed1", "slab_sizes": slab,
+                   "subdomains": len(subs), "sigma": sigma},
+        "impulses": total_imp,
+        "seconds": round(total_s, 4),
+        "gemm_tflops": round(achieved, 2) if achieved else None,
+        "gemm_frac_fp64_peak": round(achieved / PEAK["fp64_mfma_tflops"], 4) if achieved else None,
+        "flop_done": total_flop,
+        "flop_reference_algorithm": ref_flop,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

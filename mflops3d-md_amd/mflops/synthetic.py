@@ -51,6 +51,37 @@ def normal(seed: int, count: int, offset: int = 0) -> np.ndarray:
     return z[:count]
 
 
+def subdomain_layout(ndom=(4, 4, 2), sizes=(48, 64, 96), seed: int = 1):
+    """Influence-matrix decomposition of [0,1]^3 into ndom slabs per axis with
+    point counts drawn per slab from `sizes` (so faces conform).  Each
+    subdomain grid is the reference's asin/cos stretching on its slab
+    (class_mesh.f90:156-200: per-subdomain split of [xa,xb], then stretch).
+
+    Returns (slab_sizes, subdomains) with subdomains a list of
+    dict(index=(a,b,c), grids=(gx,gy,gz), interfaces=[face ids]).
+    """
+    u = uniform01(seed, sum(ndom))
+    draws = [sizes[int(v * len(sizes))] for v in u]
+    slab = [draws[0:ndom[0]], draws[ndom[0]:ndom[0] + ndom[1]], draws[ndom[0] + ndom[1]:]]
+    subs = []
+    for c in range(ndom[2]):
+        for b in range(ndom[1]):
+            for a in range(ndom[0]):
+                idx = (a, b, c)
+                grids = []
+                faces = []
+                for ax in range(3):
+                    k = idx[ax]
+                    lo, hi = k / ndom[ax], (k + 1) / ndom[ax]
+                    grids.append(grid_asin(slab[ax][k], lo, hi))
+                    if k > 0:
+                        faces.append(2 * ax)
+                    if k < ndom[ax] - 1:
+                        faces.append(2 * ax + 1)
+                subs.append({"index": idx, "grids": tuple(grids), "interfaces": faces})
+    return slab, subs
+
+
 def helmholtz_256_problem(n: int = 256, stretch: float = 2.0, seed: int = 2, modes: int = 8,
                           kmax: int = 16, noise: float = 1e-3):
     """Single-domain n^3 box [0,1]^3, tanh-stretched in y, uniform x and z;
