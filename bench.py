@@ -214,6 +214,7 @@ def run_influence(args):
     chunk = 4096
     total_imp, total_s, total_flop, ref_flop = 0, 0.0, 0.0, 0.0
     gemm_ms, gemm_work = 0.0, 0.0
+    parts = {}
     log("influence: slab sizes x=%s y=%s z=%s" % tuple(slab))
     for si, sd in enumerate(subs):
         gx, gy, gz = sd["grids"]
@@ -242,6 +243,8 @@ def run_influence(args):
         sc.profile(False)
         gemm_ms += prof["transform_fp64"][0]
         gemm_work += prof["transform_fp64"][2]
+        for k in ("fold", "transform_fp64", "faces"):
+            parts[k] = parts.get(k, 0.0) + prof[k][0]
         total_imp += nimp
         total_s += dt
         per = 2.0 * n1 * n2 * n3 * (n1 + n2 + n3)
@@ -266,6 +269,9 @@ def run_influence(args):
         "gemm_frac_fp64_peak": round(achieved / PEAK["fp64_mfma_tflops"], 4) if achieved else None,
         "flop_done": total_flop,
         "flop_reference_algorithm": ref_flop,
+        "kernel_seconds": {"spectral_impulse": round(parts.get("fold", 0) / 1e3, 4),
+                           "backward_transforms": round(parts.get("transform_fp64", 0) / 1e3, 4),
+                           "face_derivatives": round(parts.get("faces", 0) / 1e3, 4)},
     }
     print(json.dumps(out), flush=True)
 

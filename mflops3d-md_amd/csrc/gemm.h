@@ -77,8 +77,10 @@ __global__ void __launch_bounds__(256)
   const long n0 = (long)blockIdx.y * BN;
   A += (long)blockIdx.z * sA;
   if (EPI == EPI_STORE || EPI == EPI_DIVIDE) C += (long)blockIdx.z * sC;
-  const int wm = (BM == 256) ? wave * 64 : 0;
-  const int wn = (BM == 256) ? 0 : wave * 64;
+  // four 64x64 wave tiles arranged (BM/64) x (BN/64)
+  constexpr int WM = BM / 64;
+  const int wm = (wave % WM) * 64;
+  const int wn = (wave / WM) * 64;
 
   acc_t acc[4][4];
 #pragma unroll

@@ -548,13 +548,40 @@ namespace {
 using mf::EpiArgs;
 
 template <typename T, typename TA, typename TB, typename TC, int BM, int BN, int EPI>
-void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, long ldb, TC *C,
+void launch_tile(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, long ldb, TC *C,
                  long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
                  hipStream_t st) {
   dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
   ProfScope ps(h, sizeof(T) == 8 ? P_T64 : P_T32, 2.0 * M * N * K * batch, st);
   hipLaunchKernelGGL((mf::gemm_mode_kernel<T, TA, TB, TC, BM, BN, EPI>), grid, dim3(256), 0, st, A,
                      lda, sA, B, ldb, C, ldc, sC, M, N, K, ep);
+}
+
+// Block tile by the eigenvector dimension n (M for the x-transform, whose
+// eigenvector matrix is the left operand; N for y/z): the tile side along n
+// is the smallest of 64/128/256 covering it, so every field element is read
+// once and few MFMA rows are padding.  WLEFT selects the orientation.
+template <typename T, typename TA, typename TB, typename TC, bool WLEFT, int EPI>
+void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, long ldb, TC *C,
+                 long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
+                 hipStream_t st) {
+  const int n = WLEFT ? M : N;
+  const int t = n <= 64 ? 64 : (n <= 128 ? 128 : 256);
+  if (WLEFT) {
+    if (t == 64)
+      launch_tile<T, TA, TB, TC, 64, 256, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    else if (t == 128)
+      launch_tile<T, TA, TB, TC, 128, 128, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    else
+      launch_tile<T, TA, TB, TC, 256, 64, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+  } else {
+    if (t == 64)
+      launch_tile<T, TA, TB, TC, 256, 64, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    else if (t == 128)
+      launch_tile<T, TA, TB, TC, 128, 128, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    else
+      launch_tile<T, TA, TB, TC, 64, 256, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+  }
 }
 
 // Six transforms: in (interior, type TIn) -> u interior (scatter or add).
@@ -576,20 +603,20 @@ void transforms(mf_solver3d *h, const TIn *in, T *w1, T *w2, const T *Wx, const 
   ep.nx = h->nx;
   ep.ny = h->ny;
   // forward X: w1 = Wx . in
-  launch_gemm<T, T, TIn, T, 256, 64, mf::EPI_STORE>(h, Wx, n1, 0, in, n1, w1, n1, 0, n1, n2 * n3, n1, 1, ep, st);
+  launch_gemm<T, T, TIn, T, true, mf::EPI_STORE>(h, Wx, n1, 0, in, n1, w1, n1, 0, n1, n2 * n3, n1, 1, ep, st);
   // forward Y: w2_k = w1_k . Wyt
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(h, w1, n1, n12, Wyt, n2, w2, n1, n12, n1, n2, n2, n3, ep, st);
+  launch_gemm<T, T, T, T, false, mf::EPI_STORE>(h, w1, n1, n12, Wyt, n2, w2, n1, n12, n1, n2, n2, n3, ep, st);
   // forward Z with the eigenvalue divide: w1 = (w2 . Wzt) / lambda
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_DIVIDE>(h, w2, n12, 0, Wzt, n3, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  launch_gemm<T, T, T, T, false, mf::EPI_DIVIDE>(h, w2, n12, 0, Wzt, n3, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
   // backward Z: w2 = w1 . Vzt
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(h, w1, n12, 0, Vzt, n3, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  launch_gemm<T, T, T, T, false, mf::EPI_STORE>(h, w1, n12, 0, Vzt, n3, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
   // backward Y: w1_k = w2_k . Vyt
-  launch_gemm<T, T, T, T, 64, 256, mf::EPI_STORE>(h, w2, n1, n12, Vyt, n2, w1, n1, n12, n1, n2, n2, n3, ep, st);
+  launch_gemm<T, T, T, T, false, mf::EPI_STORE>(h, w2, n1, n12, Vyt, n2, w1, n1, n12, n1, n2, n2, n3, ep, st);
   // backward X into u's interior
   if (accumulate)
-    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER_ADD>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+    launch_gemm<T, T, T, double, true, mf::EPI_SCATTER_ADD>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
   else
-    launch_gemm<T, T, T, double, 256, 64, mf::EPI_SCATTER>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+    launch_gemm<T, T, T, double, true, mf::EPI_SCATTER>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
 }
 
 int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy,
@@ -850,11 +877,11 @@ extern "C" int mf_influence_impulses(mf_solver3d *h, const mf_der *dx, const mf_
                          0, st, W1, ia);
     }
     // backward transforms, batched over the impulses
-    launch_gemm<double, double, double, double, 64, 256, mf::EPI_STORE>(
+    launch_gemm<double, double, double, double, false, mf::EPI_STORE>(
         h, W1, n12, N, h->Vzt, n3, W2, n12, N, (int)n12, n3, n3, nb, ep, st);
-    launch_gemm<double, double, double, double, 64, 256, mf::EPI_STORE>(
+    launch_gemm<double, double, double, double, false, mf::EPI_STORE>(
         h, W2, n1, n12, h->Vyt, n2, W1, n1, n12, n1, n2, n2, n3 * nb, ep, st);
-    launch_gemm<double, double, double, double, 256, 64, mf::EPI_STORE>(
+    launch_gemm<double, double, double, double, true, mf::EPI_STORE>(
         h, h->Vx, n1, 0, W1, n1, W2, n1, 0, n1, (int)((long)n2 * n3 * nb), n1, 1, ep, st);
     mf::FaceDerArgs fa;
     fa.n1 = n1; fa.n2 = n2; fa.n3 = n3; fa.face = face; fa.first = first + b0;
