@@ -26,6 +26,7 @@
 #include "gemm.h"
 #include "influence.h"
 #include "internal.h"
+#include "residual.h"
 #include "mflops/mf.h"
 
 namespace mf {
@@ -629,56 +630,66 @@ int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy
   return MF_OK;
 }
 
+mf::BandRows rows(const mf::Band &b) { return mf::BandRows{b.c0, b.w}; }
+
 // r' = b' - B u with B the homogeneous operator in multiplied-through form;
 // u is read in place from the full array (interior at offset (1,1,1)).
-// Work: T3 = s1, T4 = s2 (z-pass), T5 = t1, T1 = t2 (y-pass); r' -> T3.
+// Work: T4 = a, T5 = c (plane pass); r' -> rp.
 int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime, double *rp,
              double *norm2_dev, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
   const long ldj = h->nx, ldk = (long)h->nx * h->ny;
-  const double *u0 = u + 1 + ldj + ldk;
-  const long il = n1, ik = (long)n1 * n2;
-  mf::Band none = {nullptr, nullptr};
-  const double *dnull = nullptr;
-  double *wnull = nullptr;
   ProfScope ps(h, P_RES, 24.0 * N, st);  // algorithmic: read u and b', write r'
-  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3), blk(64, 4);
-  const long nb = (long)grid.x * grid.y * grid.z;
-  hipLaunchKernelGGL((mf::band_kernel<2, 1>), grid, blk, 0, st, u0, ldj, ldk, dnull,
-                     h->T3, h->T4, n1, n2, n3, h->M[2], h->R[2], none, sigma, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<1, 2>), grid, blk, 0, st, (const double *)h->T3,
-                     il, ik, (const double *)h->T4, h->T5, h->T1, n1, n2, n3, h->M[1], h->R[1], none,
-                     0.0, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<0, 3>), grid, blk, 0, st, (const double *)h->T1,
-                     il, ik, (const double *)h->T5, rp, wnull, n1, n2, n3, h->R[0], h->M[0], none,
-                     0.0, bprime, h->part);
-  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part, nb,
-                     norm2_dev);
+  mf::PlaneArgs pa;
+  std::memset(&pa, 0, sizeof(pa));
+  pa.n1 = n1; pa.n2 = n2; pa.n3 = n3;
+  pa.u = u + 1 + ldj + ldk; pa.ldj = ldj; pa.ldk = ldk;
+  pa.a = h->T4; pa.c = h->T5;
+  pa.Mx = rows(h->M[0]); pa.Rx = rows(h->R[0]); pa.My = rows(h->M[1]); pa.Ry = rows(h->R[1]);
+  pa.sigma = sigma;
+  hipLaunchKernelGGL(mf::plane_kernel<0>,
+                     dim3((n1 + mf::PT_X - 1) / mf::PT_X, (n2 + mf::PT_Y - 1) / mf::PT_Y, n3),
+                     dim3(64, 4), 0, st, pa);
+  mf::ZArgs za;
+  za.n1 = n1; za.n2 = n2; za.n3 = n3;
+  za.a = h->T4; za.c = h->T5; za.bprime = bprime; za.out = rp; za.part = h->part;
+  za.Mz = rows(h->M[2]); za.Rz = rows(h->R[2]);
+  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3);
+  hipLaunchKernelGGL(mf::zpass_kernel<0>, grid, dim3(64, 4), 0, st, za);
+  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
+                     (long)grid.x * grid.y * grid.z, norm2_dev);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
 
-// b' = lhs_x lhs_y lhs_z g_f with ||b'||^2; g_f in T1, b' -> T2.
-int multiply_through(mf_solver3d *h, double *norm2_dev, hipStream_t st) {
+// b' = lhs_x lhs_y lhs_z g_f with ||b'||^2, the BC fold of g done on the fly
+// (g_f is never stored); b' -> T2.
+int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *bcy,
+              const double *bcz, double *norm2_dev, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
-  const long il = n1, ik = (long)n1 * n2;
-  mf::Band none = {nullptr, nullptr};
-  const double *dnull = nullptr;
-  double *wnull = nullptr;
-  ProfScope ps(h, P_MULT, 16.0 * N, st);  // algorithmic: read g_f, write b'
-  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3), blk(64, 4);
-  const long nb = (long)grid.x * grid.y * grid.z;
-  hipLaunchKernelGGL((mf::band_kernel<2, 0>), grid, blk, 0, st, (const double *)h->T1,
-                     il, ik, dnull, h->T3, wnull, n1, n2, n3, h->M[2], none, none, 0.0, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<1, 0>), grid, blk, 0, st, (const double *)h->T3,
-                     il, ik, dnull, h->T4, wnull, n1, n2, n3, h->M[1], none, none, 0.0, dnull, wnull);
-  hipLaunchKernelGGL((mf::band_kernel<0, 4>), grid, blk, 0, st, (const double *)h->T4,
-                     il, ik, dnull, h->T2, wnull, n1, n2, n3, h->M[0], none, none, 0.0, dnull,
-                     h->part);
-  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part, nb,
-                     norm2_dev);
+  const long ldj = h->nx, ldk = (long)h->nx * h->ny;
+  ProfScope ps(h, P_MULT, 16.0 * N, st);  // algorithmic: read g, write b'
+  mf::PlaneArgs pa;
+  std::memset(&pa, 0, sizeof(pa));
+  pa.n1 = n1; pa.n2 = n2; pa.n3 = n3;
+  pa.u = g + 1 + ldj + ldk; pa.ldj = ldj; pa.ldk = ldk;
+  pa.a = h->T4;
+  pa.Mx = rows(h->M[0]); pa.Rx = rows(h->R[0]); pa.My = rows(h->M[1]); pa.Ry = rows(h->R[1]);
+  pa.fold = h->fold; pa.bcx = bcx; pa.bcy = bcy; pa.bcz = bcz;
+  hipLaunchKernelGGL(mf::plane_kernel<1>,
+                     dim3((n1 + mf::PT_X - 1) / mf::PT_X, (n2 + mf::PT_Y - 1) / mf::PT_Y, n3),
+                     dim3(64, 4), 0, st, pa);
+  mf::ZArgs za;
+  std::memset(&za, 0, sizeof(za));
+  za.n1 = n1; za.n2 = n2; za.n3 = n3;
+  za.a = h->T4; za.out = h->T2; za.part = h->part;
+  za.Mz = rows(h->M[2]); za.Rz = rows(h->R[2]);
+  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3);
+  hipLaunchKernelGGL(mf::zpass_kernel<1>, grid, dim3(64, 4), 0, st, za);
+  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
+                     (long)grid.x * grid.y * grid.z, norm2_dev);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
@@ -711,13 +722,7 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   // ---- mixed precision: FP32 MFMA transforms, FP64 residual refinement
   if (!(tol > 0.0)) tol = 1e-12;
   if (max_iter <= 0) max_iter = 8;
-  {
-    ProfScope ps(h, P_FOLD, 16.0 * N, st);
-    hipLaunchKernelGGL(mf::fold_kernel, dim3((h->n1 + 63) / 64, (h->n2 + 3) / 4, h->n3), dim3(64, 4), 0, st, g, h->T1, h->n1, h->n2,
-                       h->n3, (const double *)h->fold, bcx, bcy, bcz);
-  }
-  HIPCHK(hipMemsetAsync(h->norms, 0, 8 * sizeof(double), st));
-  int rc = multiply_through(h, h->norms + 0, st);
+  int rc = rhs_prime(h, g, bcx, bcy, bcz, h->norms + 0, st);
   if (rc != MF_OK) return rc;
   // first low-precision solve: u_int = S(b')
   transforms<float, double>(h, h->T2, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
