@@ -42,7 +42,11 @@ enum {
   MF_ERR_NOCONV = 4    /* mixed-precision refinement did not reach tol */
 };
 
-enum { MF_FP64 = 0, MF_MIXED = 1 };
+/* MF_MIXED: FP32 MFMA transforms.  MF_MIXED_BF16X3: the same transforms with
+ * every operand split into three bf16 parts (six bf16 MFMA products, FP32
+ * accumulation: FP32-class accuracy on the bf16 matrix cores).  Both refine
+ * with the FP64 residual to tol. */
+enum { MF_FP64 = 0, MF_MIXED = 1, MF_MIXED_BF16X3 = 2 };
 
 typedef struct mf_der mf_der;
 typedef struct mf_solver3d mf_solver3d;
@@ -112,9 +116,10 @@ int mf_solver3d_destroy(mf_solver3d *sc);
 /* Solve with g(nx,ny,nz) (only the interior is read; g is not modified),
  * u(nx,ny,nz) inout (interior and the 6 faces written, the 12 edges left
  * untouched).  precision MF_FP64: the reference algorithm in FP64.
- * MF_MIXED: FP32 MFMA transforms with FP64-residual iterative refinement
- * until ||r||/||b|| <= tol (tol <= 0 selects 1e-12) or max_iter solves.
- * MIXED synchronises the stream (it reads back residual norms). */
+ * MF_MIXED / MF_MIXED_BF16X3: FP32 / split-bf16 MFMA transforms with
+ * FP64-residual iterative refinement until ||r||/||b|| <= tol (tol <= 0
+ * selects 1e-12) or max_iter solves (max_iter <= 0 selects 8); both
+ * synchronise the stream (they read back residual norms). */
 int mf_solver3d_solve(mf_solver3d *sc, const double *g, double sigma, double *u,
                       const double *bcx, const double *bcy, const double *bcz,
                       int precision, double tol, int max_iter,

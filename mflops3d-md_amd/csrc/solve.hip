@@ -18,12 +18,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "gemm.h"
+#include "gemm_bf3.h"
 #include "influence.h"
 #include "internal.h"
 #include "residual.h"
@@ -113,94 +115,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// out(p) = sum_c band(row p_axis, c) in(p with axis index c), for all interior
-// points of an (n1,n2,n3) grid; inputs are strided (base + i*1 + j*ldj + k*ldk)
-// so the full-size u can be read in place.  Two bands on one input
-// (out1 = B1 in, out2 = B2 in) or a fused pair (out2 = B2 in + B3 in2).
-// FINAL: out1 = bprime - (B1 in + B3 in2), with ||out1||^2 accumulated.
-template <int AXIS, int MODE>
-__global__ void __launch_bounds__(256)
-    band_kernel(const double *__restrict__ in, long ldj, long ldk, const double *__restrict__ in2,
-                double *__restrict__ out1, double *__restrict__ out2, int n1, int n2, int n3,
-                Band B1, Band B2, Band B3, double s2, const double *__restrict__ bprime,
-                double *__restrict__ norm2) {
-  // MODE 0: out1 = B1 in
-  // MODE 1: out1 = B1 in ; out2 = B2 in + s2 * B1 in          (z-pass)
-  // MODE 2: out1 = B1 in ; out2 = B2 in + B1 in2              (y-pass)
-  // MODE 3: out1 = bprime - (B2 in + B1 in2), norm             (x-pass)
-  // MODE 4: out1 = B1 in, norm of out1                         (b' x-pass)
-  // launch: block (64,4), grid (ceil(n1/64), ceil(n2/4), n3); the norm is
-  // reduced per block into norm2[block] (deterministic; summed afterwards).
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  const int k = blockIdx.z;
-  double contrib = 0.0;
-  if (i < n1 && j < n2) {
-    const long p = i + (long)n1 * (j + (long)n2 * k);
-    const int row = AXIS == 0 ? i : (AXIS == 1 ? j : k);
-    const long stride = AXIS == 0 ? 1 : (AXIS == 1 ? ldj : ldk);
-    const long inbase = (AXIS == 0 ? 0 : i) + (AXIS == 1 ? 0 : (long)j * ldj) +
-                        (AXIS == 2 ? 0 : (long)k * ldk);
-    const long in2base = (AXIS == 0 ? 0 : i) + (AXIS == 1 ? 0 : (long)j * n1) +
-                         (AXIS == 2 ? 0 : (long)k * n1 * n2);
-    const long stride2 = AXIS == 0 ? 1 : (AXIS == 1 ? n1 : (long)n1 * n2);
-    const int c = B1.c0[row];
-    const double *w1 = B1.w + 8 * row;
-    double a1 = 0.0;
-    if (MODE == 3) {
-      // in = t2 (uses B2 = lhs_x), in2 = t1 (uses B1 = rhs_x); both interior-contiguous
-      const int c2 = B2.c0[row];
-      const double *w2 = B2.w + 8 * row;
-      double a2 = 0.0;
-#pragma unroll
-      for (int m = 0; m < 7; ++m) {
-        a1 = fma(w1[m], in2[in2base + (long)(c + m) * stride2], a1);
-        a2 = fma(w2[m], in[inbase + (long)(c2 + m) * stride], a2);
-      }
-      const double r = bprime[p] - (a1 + a2);
-      out1[p] = r;
-      contrib = r * r;
-    } else {
-#pragma unroll
-      for (int m = 0; m < 7; ++m) a1 = fma(w1[m], in[inbase + (long)(c + m) * stride], a1);
-      if (MODE == 4) {
-        out1[p] = a1;
-        contrib = a1 * a1;
-      } else {
-        out1[p] = a1;
-      }
-      if (MODE == 1) {
-        const int c2 = B2.c0[row];
-        const double *w2 = B2.w + 8 * row;
-        double a2 = 0.0;
-#pragma unroll
-        for (int m = 0; m < 7; ++m) a2 = fma(w2[m], in[inbase + (long)(c2 + m) * stride], a2);
-        out2[p] = a2 + s2 * a1;
-      } else if (MODE == 2) {
-        const int c2 = B2.c0[row];
-        const double *w2 = B2.w + 8 * row;
-        double a2 = 0.0, a3 = 0.0;
-#pragma unroll
-        for (int m = 0; m < 7; ++m) {
-          a2 = fma(w2[m], in[inbase + (long)(c2 + m) * stride], a2);
-          a3 = fma(w1[m], in2[in2base + (long)(c + m) * stride2], a3);
-        }
-        out2[p] = a2 + a3;
-      }
-    }
-  }
-  if (MODE == 3 || MODE == 4) {
-    // wavefront reduction, then the 4 wave sums in fixed order
-    __shared__ double wsum[4];
-    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
-    if (threadIdx.x == 0) wsum[threadIdx.y] = contrib;
-    __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-      norm2[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
-          (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
-  }
-}
-
 // out[0] = sum of n partial sums (one block of 1024 threads, fixed order).
 __global__ void __launch_bounds__(1024) sum_kernel(const double *__restrict__ part, long n,
                                                    double *__restrict__ out) {
@@ -231,6 +145,11 @@ struct mf_solver3d {
   double *Wx = nullptr, *Vx = nullptr, *Wyt = nullptr, *Vyt = nullptr, *Wzt = nullptr, *Vzt = nullptr;
   float *Wx32 = nullptr, *Vx32 = nullptr, *Wyt32 = nullptr, *Vyt32 = nullptr, *Wzt32 = nullptr,
         *Vzt32 = nullptr;  // mixed: W' = inv(V) inv(lhs)
+  // the same six matrices split into three bf16 parts, [part][row][k]
+  // (row = the eigenvector-dimension index, zero-padded to wrows[axis] rows
+  // and kp[axis] columns)
+  __bf16 *Wsp[3] = {nullptr, nullptr, nullptr}, *Vsp[3] = {nullptr, nullptr, nullptr};
+  int kp[3] = {0, 0, 0}, wrows[3] = {0, 0, 0};
   double *lam = nullptr;   // lx(n1) ly(n2) lz(n3)
   double *fold = nullptr;  // fx1 fx2 fy1 fy2 fz1 fz2
   double *neum = nullptr;  // 3 x 14
@@ -239,8 +158,10 @@ struct mf_solver3d {
   double *bandw = nullptr; // [axis][M|R] (n,8)
   mf::Band M[3], R[3];
   double *T1 = nullptr, *T2 = nullptr;  // interior FP64 work
-  double *T3 = nullptr, *T4 = nullptr, *T5 = nullptr;  // residual work (mixed)
   float *F1 = nullptr, *F2 = nullptr;   // interior FP32 work (mixed)
+  float *R32 = nullptr;                 // b' / r' in FP32 (mixed)
+  double *ebd = nullptr;                // ex1 ex2 ey1 ey2 ez1 ez2: P-folded boundary rows
+  double *fbf = nullptr;                // face products fbx fby fbz (mixed)
   double *norms = nullptr;              // device scalars
   double *part = nullptr;               // per-block partial sums of norms
   double *hnorms = nullptr;             // pinned host mirror
@@ -291,27 +212,58 @@ struct ProfScope {
   }
 };
 
+// Side of the GEMM block tile along an eigenvector dimension of n: the
+// smallest of 64/128/256 covering it (256 beyond).
+inline int gemm_tile(int n) { return n <= 64 ? 64 : (n <= 128 ? 128 : 256); }
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+inline uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+inline float bf16_float(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+// x = p0 + p1 + p2 in bf16 (the device split3 in gemm_bf3.h, on the host)
+inline void bf16_split3(float x, uint16_t p[3]) {
+  p[0] = bf16_bits(x);
+  const float r1 = x - bf16_float(p[0]);
+  p[1] = bf16_bits(r1);
+  const float r2 = r1 - bf16_float(p[1]);
+  p[2] = bf16_bits(r2);
+}
+
 void transpose(int n, const std::vector<double> &a, std::vector<double> &t) {
   t.resize((size_t)n * n);
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) t[j + (size_t)n * i] = a[i + (size_t)n * j];
 }
 
-// rows of a dense banded matrix as (start column, 7 weights)
+// Rows of a dense banded matrix as (start column, BAND_W weights).  The
+// window of row i is centred, [i-2, i+2], where the row's nonzeros allow, and
+// always lies in [i-4, i+4] inside the line; starts are non-decreasing in i
+// (the residual kernel's halo and plane range rely on both).
 bool to_band(int n, const std::vector<double> &a, std::vector<int> &c0, std::vector<double> &w) {
+  constexpr int W = mf::BAND_W, R = mf::RZ_H;
   c0.assign(n, 0);
   w.assign((size_t)8 * n, 0.0);
   for (int i = 0; i < n; ++i) {
-    int lo = n, hi = -1;
+    int first = n, last = -1;
     for (int j = 0; j < n; ++j)
-      if (a[i + (size_t)n * j] != 0.0) { lo = std::min(lo, j); hi = std::max(hi, j); }
-    if (hi < 0) { lo = hi = i; }
-    if (hi - lo >= 7) return false;
-    lo = std::min(lo, n - 7);  // keep the 7-wide window inside the line
-    lo = std::max(lo, 0);
-    if (hi - lo >= 7) return false;
+      if (a[i + (size_t)n * j] != 0.0) { first = std::min(first, j); last = std::max(last, j); }
+    if (last < 0) first = last = i;
+    const int lo_min = std::max(std::max(0, last - (W - 1)), i - R);
+    const int lo_max = std::min(std::min(first, n - W), i);
+    if (lo_min > lo_max) return false;
+    const int lo = std::min(std::max(i - W / 2, lo_min), lo_max);
+    if (i > 0 && lo < c0[i - 1]) return false;
     c0[i] = lo;
-    for (int m = 0; m < 7; ++m) w[(size_t)8 * i + m] = a[i + (size_t)n * (lo + m)];
+    for (int m = 0; m < W; ++m) w[(size_t)8 * i + m] = a[i + (size_t)n * (lo + m)];
   }
   return true;
 }
@@ -434,7 +386,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
       transpose(m, wp, Wp[a]);
     }
   }
-  std::vector<double> lam, fold, neum;
+  std::vector<double> lam, fold, neum, ebd;
   for (int a = 0; a < 3; ++a) lam.insert(lam.end(), h->ax[a].eigen.begin(), h->ax[a].eigen.end());
   for (int a = 0; a < 3; ++a) {
     const mf::Solver1D &s = h->ax[a];
@@ -448,6 +400,14 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     }
     fold.insert(fold.end(), f1.begin(), f1.end());
     fold.insert(fold.end(), f2.begin(), f2.end());
+    // lhs f1 and lhs f2: the boundary columns of rhs (two rows each)
+    std::vector<double> e1(m, 0.0), e2(m, 0.0);
+    e1[0] = DR(0, 0);
+    e1[1] = DR(1, 0);
+    e2[m - 2] = DR(m - 2, 7);
+    e2[m - 1] = DR(m - 1, 6);
+    ebd.insert(ebd.end(), e1.begin(), e1.end());
+    ebd.insert(ebd.end(), e2.begin(), e2.end());
     neum.insert(neum.end(), s.neum.begin(), s.neum.end());
   }
   std::vector<int> bandc;
@@ -457,13 +417,13 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     std::vector<double> w;
     if (!to_band(n[a], h->ax[a].lhs, c, w) ) {
       delete h;
-      return mf::fail(MF_ERR_SETUP, "mf_solver3d_create: lhs is wider than 7 bands");
+      return mf::fail(MF_ERR_SETUP, "mf_solver3d_create: lhs rows do not fit 5-wide windows");
     }
     bandc.insert(bandc.end(), c.begin(), c.end());
     bandw.insert(bandw.end(), w.begin(), w.end());
     if (!to_band(n[a], h->ax[a].rhs, c, w)) {
       delete h;
-      return mf::fail(MF_ERR_SETUP, "mf_solver3d_create: rhs is wider than 7 bands");
+      return mf::fail(MF_ERR_SETUP, "mf_solver3d_create: rhs rows do not fit 5-wide windows");
     }
     bandc.insert(bandc.end(), c.begin(), c.end());
     bandw.insert(bandw.end(), w.begin(), w.end());
@@ -471,7 +431,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
   // device layout: one allocation carved into 256-byte aligned arrays; the
   // carve runs twice, first to size the allocation, then to assign pointers
   const size_t N = (size_t)n[0] * n[1] * n[2];
-  const size_t nparts = (size_t)((n[0] + 63) / 64) * ((n[1] + 3) / 4) * n[2];
+  const size_t nparts = (size_t)((n[0] + mf::RZ_TX - 1) / mf::RZ_TX) *
+                        ((n[1] + mf::RZ_TY - 1) / mf::RZ_TY) * ((n[2] + mf::RZ_CZ - 1) / mf::RZ_CZ);
   double *dW[3], *dV[3];
   float *fW[3], *fV[3];
   auto carve = [&](double *base) -> size_t {
@@ -492,11 +453,22 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->bandw = take(bandw.size());
     h->norms = take(8);
     h->part = take(nparts);
-    h->T1 = take(N); h->T2 = take(N); h->T3 = take(N); h->T4 = take(N); h->T5 = take(N);
+    h->T1 = take(N); h->T2 = take(N);
     h->F1 = (float *)take((N + 1) / 2);
     h->F2 = (float *)take((N + 1) / 2);
+    h->R32 = (float *)take((N + 1) / 2);
+    h->ebd = take(ebd.size());
+    h->fbf = take(2 * ((size_t)n[1] * n[2] + (size_t)n[0] * n[2] + (size_t)n[0] * n[1]));
     h->bandc = (int *)take((bandc.size() + 1) / 2);
     h->bctd = (int *)take(4);
+    for (int a = 0; a < 3; ++a) {
+      h->kp[a] = (n[a] + 31) / 32 * 32;
+      const int t = gemm_tile(n[a]);
+      h->wrows[a] = (n[a] + t - 1) / t * t;
+      const size_t cnt = (3 * (size_t)h->wrows[a] * h->kp[a] + 3) / 4;  // bf16 -> doubles
+      h->Wsp[a] = (__bf16 *)take(cnt);
+      h->Vsp[a] = (__bf16 *)take(cnt);
+    }
     return off;
   };
   const size_t nd = carve(nullptr);
@@ -523,8 +495,27 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     if (e == hipSuccess) e = upf(fW[a], Wp[a]);
     if (e == hipSuccess) e = upf(fV[a], V[a]);
   }
+  // three-part bf16 splits: [part][row][k]; the x matrices are the left
+  // operand (row = m of W(m,k)), the stored y/z transposes the right one
+  // (row = n of B(k,n))
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    const int m = n[a], kp = h->kp[a], wr = h->wrows[a];
+    for (int which = 0; which < 2 && e == hipSuccess; ++which) {
+      const std::vector<double> &src = which == 0 ? Wp[a] : V[a];
+      std::vector<uint16_t> sp((size_t)3 * wr * kp, 0);
+      for (int r = 0; r < m; ++r)
+        for (int k = 0; k < m; ++k) {
+          const double v = (a == 0) ? src[r + (size_t)m * k] : src[k + (size_t)m * r];
+          uint16_t p[3];
+          bf16_split3((float)v, p);
+          for (int q = 0; q < 3; ++q) sp[((size_t)q * wr + r) * kp + k] = p[q];
+        }
+      e = up(which == 0 ? (void *)h->Wsp[a] : (void *)h->Vsp[a], sp.data(), sp.size() * 2);
+    }
+  }
   if (e == hipSuccess) e = up(h->lam, lam.data(), lam.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->fold, fold.data(), fold.size() * sizeof(double));
+  if (e == hipSuccess) e = up(h->ebd, ebd.data(), ebd.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->neum, neum.data(), neum.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->bandw, bandw.data(), bandw.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->bandc, bandc.data(), bandc.size() * sizeof(int));
@@ -566,8 +557,7 @@ template <typename T, typename TA, typename TB, typename TC, bool WLEFT, int EPI
 void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, long ldb, TC *C,
                  long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
                  hipStream_t st) {
-  const int n = WLEFT ? M : N;
-  const int t = n <= 64 ? 64 : (n <= 128 ? 128 : 256);
+  const int t = gemm_tile(WLEFT ? M : N);
   if (WLEFT) {
     if (t == 64)
       launch_tile<T, TA, TB, TC, 64, 256, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
@@ -620,6 +610,72 @@ void transforms(mf_solver3d *h, const TIn *in, T *w1, T *w2, const T *Wx, const 
     launch_gemm<T, T, T, double, true, mf::EPI_SCATTER>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
 }
 
+// ---- split-bf16 transforms (MF_MIXED_BF16X3)
+template <typename TF, typename TC, int BM, int BN, bool WLEFT, int EPI>
+void launch_bf3_tile(mf_solver3d *h, const TF *F, long ldf, long sF, const __bf16 *W, int KP,
+                     int WROWS, TC *C, long ldc, long sC, int M, int N, int K, int batch,
+                     const EpiArgs &ep, hipStream_t st) {
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  ProfScope ps(h, P_T32, 2.0 * M * N * K * batch, st);
+  hipLaunchKernelGGL((mf::gemm_bf3_kernel<TF, TC, BM, BN, WLEFT, EPI>), grid, dim3(256), 0, st, F,
+                     ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, ep);
+}
+
+// Tile choice as launch_gemm: the side along the eigenvector dimension is the
+// smallest of 64/128/256 that covers it.
+template <typename TF, typename TC, bool WLEFT, int EPI>
+void launch_bf3(mf_solver3d *h, const TF *F, long ldf, long sF, const __bf16 *W, int KP, int WROWS,
+                TC *C, long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
+                hipStream_t st) {
+  const int t = gemm_tile(WLEFT ? M : N);
+  if (WLEFT) {
+    if (t == 64)
+      launch_bf3_tile<TF, TC, 64, 256, WLEFT, EPI>(h, F, ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, batch, ep, st);
+    else if (t == 128)
+      launch_bf3_tile<TF, TC, 128, 128, WLEFT, EPI>(h, F, ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, batch, ep, st);
+    else
+      launch_bf3_tile<TF, TC, 256, 64, WLEFT, EPI>(h, F, ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, batch, ep, st);
+  } else {
+    if (t == 64)
+      launch_bf3_tile<TF, TC, 256, 64, WLEFT, EPI>(h, F, ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, batch, ep, st);
+    else if (t == 128)
+      launch_bf3_tile<TF, TC, 128, 128, WLEFT, EPI>(h, F, ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, batch, ep, st);
+    else
+      launch_bf3_tile<TF, TC, 64, 256, WLEFT, EPI>(h, F, ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, batch, ep, st);
+  }
+}
+
+// The six transforms of transforms<float, double> with the pre-split
+// matrices W' = inv(V) inv(lhs) and V.
+void transforms_bf3(mf_solver3d *h, const float *in, float *w1, float *w2, double sigma,
+                    double *u, bool accumulate, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  const long n12 = (long)n1 * n2;
+  const int kx = h->kp[0], ky = h->kp[1], kz = h->kp[2];
+  const int rx = h->wrows[0], ry = h->wrows[1], rz = h->wrows[2];
+  EpiArgs ep;
+  std::memset(&ep, 0, sizeof(ep));
+  ep.lx = h->lam;
+  ep.ly = h->lam + n1;
+  ep.lz = h->lam + n1 + n2;
+  ep.sigma = sigma;
+  ep.n1 = n1;
+  ep.n2 = n2;
+  ep.u = u;
+  ep.nx = h->nx;
+  ep.ny = h->ny;
+  using mf::EPI_STORE;
+  launch_bf3<float, float, true, EPI_STORE>(h, in, n1, 0, h->Wsp[0], kx, rx, w1, n1, 0, n1, n2 * n3, n1, 1, ep, st);
+  launch_bf3<float, float, false, EPI_STORE>(h, w1, n1, n12, h->Wsp[1], ky, ry, w2, n1, n12, n1, n2, n2, n3, ep, st);
+  launch_bf3<float, float, false, mf::EPI_DIVIDE>(h, w2, n12, 0, h->Wsp[2], kz, rz, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  launch_bf3<float, float, false, EPI_STORE>(h, w1, n12, 0, h->Vsp[2], kz, rz, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
+  launch_bf3<float, float, false, EPI_STORE>(h, w2, n1, n12, h->Vsp[1], ky, ry, w1, n1, n12, n1, n2, n2, n3, ep, st);
+  if (accumulate)
+    launch_bf3<float, double, true, mf::EPI_SCATTER_ADD>(h, w1, n1, 0, h->Vsp[0], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+  else
+    launch_bf3<float, double, true, mf::EPI_SCATTER>(h, w1, n1, 0, h->Vsp[0], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
+}
+
 int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy,
                  const double *bcz, hipStream_t st) {
   const long nf = 2 * ((long)h->n2 * h->n3 + (long)h->n1 * h->n3 + (long)h->n1 * h->n2);
@@ -632,66 +688,64 @@ int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy
 
 mf::BandRows rows(const mf::Band &b) { return mf::BandRows{b.c0, b.w}; }
 
-// r' = b' - B u with B the homogeneous operator in multiplied-through form;
-// u is read in place from the full array (interior at offset (1,1,1)).
-// Work: T4 = a, T5 = c (plane pass); r' -> rp.
-int residual(mf_solver3d *h, const double *u, double sigma, const double *bprime, double *rp,
-             double *norm2_dev, hipStream_t st) {
+mf::ResArgs res_args(mf_solver3d *h, const double *u, const double *g, double sigma) {
+  mf::ResArgs a;
+  std::memset(&a, 0, sizeof(a));
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
-  const long N = (long)n1 * n2 * n3;
   const long ldj = h->nx, ldk = (long)h->nx * h->ny;
-  ProfScope ps(h, P_RES, 24.0 * N, st);  // algorithmic: read u and b', write r'
-  mf::PlaneArgs pa;
-  std::memset(&pa, 0, sizeof(pa));
-  pa.n1 = n1; pa.n2 = n2; pa.n3 = n3;
-  pa.u = u + 1 + ldj + ldk; pa.ldj = ldj; pa.ldk = ldk;
-  pa.a = h->T4; pa.c = h->T5;
-  pa.Mx = rows(h->M[0]); pa.Rx = rows(h->R[0]); pa.My = rows(h->M[1]); pa.Ry = rows(h->R[1]);
-  pa.sigma = sigma;
-  hipLaunchKernelGGL(mf::plane_kernel<0>,
-                     dim3((n1 + mf::PT_X - 1) / mf::PT_X, (n2 + mf::PT_Y - 1) / mf::PT_Y, n3),
-                     dim3(64, 4), 0, st, pa);
-  mf::ZArgs za;
-  za.n1 = n1; za.n2 = n2; za.n3 = n3;
-  za.a = h->T4; za.c = h->T5; za.bprime = bprime; za.out = rp; za.part = h->part;
-  za.Mz = rows(h->M[2]); za.Rz = rows(h->R[2]);
-  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3);
-  hipLaunchKernelGGL(mf::zpass_kernel<0>, grid, dim3(64, 4), 0, st, za);
+  a.n1 = n1; a.n2 = n2; a.n3 = n3;
+  a.u = u ? u + 1 + ldj + ldk : nullptr;
+  a.g = g + 1 + ldj + ldk;
+  a.ldj = ldj; a.ldk = ldk;
+  a.sigma = sigma;
+  a.Mx = rows(h->M[0]); a.Rx = rows(h->R[0]); a.My = rows(h->M[1]); a.Ry = rows(h->R[1]);
+  a.Mz = rows(h->M[2]); a.Rz = rows(h->R[2]);
+  a.ex = h->ebd; a.ey = h->ebd + 2 * n1; a.ez = h->ebd + 2 * (n1 + n2);
+  a.fbx = h->fbf;
+  a.fby = a.fbx + 2L * n2 * n3;
+  a.fbz = a.fby + 2L * n1 * n3;
+  a.out = h->R32;
+  a.part = h->part;
+  return a;
+}
+
+template <bool RES>
+int launch_residual(mf_solver3d *h, const mf::ResArgs &a, double *norm2_dev, hipStream_t st) {
+  const dim3 grid((h->n1 + mf::RZ_TX - 1) / mf::RZ_TX, (h->n2 + mf::RZ_TY - 1) / mf::RZ_TY,
+                  (h->n3 + mf::RZ_CZ - 1) / mf::RZ_CZ);
+  hipLaunchKernelGGL(mf::residual_kernel<RES>, grid, dim3(64, 4), 0, st, a);
   hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
                      (long)grid.x * grid.y * grid.z, norm2_dev);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
 
-// b' = lhs_x lhs_y lhs_z g_f with ||b'||^2, the BC fold of g done on the fly
-// (g_f is never stored); b' -> T2.
+// r' = P g_f - B u (FP32 copy -> R32) and ||r'||^2 -> norm2_dev, in one pass
+// over u and g; the face products must be current (rhs_prime).
+int residual(mf_solver3d *h, const double *u, const double *g, double sigma, double *norm2_dev,
+             hipStream_t st) {
+  const long N = (long)h->n1 * h->n2 * h->n3;
+  ProfScope ps(h, P_RES, 20.0 * N, st);  // algorithmic: read u and g, write r' (FP32)
+  return launch_residual<true>(h, res_args(h, u, g, sigma), norm2_dev, st);
+}
+
+// b' = P g_f (FP32 copy -> R32) with ||b'||^2, and the face products of the
+// boundary fold for the residuals that follow.
 int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *bcy,
               const double *bcz, double *norm2_dev, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
-  const long ldj = h->nx, ldk = (long)h->nx * h->ny;
-  ProfScope ps(h, P_MULT, 16.0 * N, st);  // algorithmic: read g, write b'
-  mf::PlaneArgs pa;
-  std::memset(&pa, 0, sizeof(pa));
-  pa.n1 = n1; pa.n2 = n2; pa.n3 = n3;
-  pa.u = g + 1 + ldj + ldk; pa.ldj = ldj; pa.ldk = ldk;
-  pa.a = h->T4;
-  pa.Mx = rows(h->M[0]); pa.Rx = rows(h->R[0]); pa.My = rows(h->M[1]); pa.Ry = rows(h->R[1]);
-  pa.fold = h->fold; pa.bcx = bcx; pa.bcy = bcy; pa.bcz = bcz;
-  hipLaunchKernelGGL(mf::plane_kernel<1>,
-                     dim3((n1 + mf::PT_X - 1) / mf::PT_X, (n2 + mf::PT_Y - 1) / mf::PT_Y, n3),
-                     dim3(64, 4), 0, st, pa);
-  mf::ZArgs za;
-  std::memset(&za, 0, sizeof(za));
-  za.n1 = n1; za.n2 = n2; za.n3 = n3;
-  za.a = h->T4; za.out = h->T2; za.part = h->part;
-  za.Mz = rows(h->M[2]); za.Rz = rows(h->R[2]);
-  const dim3 grid((n1 + 63) / 64, (n2 + 3) / 4, n3);
-  hipLaunchKernelGGL(mf::zpass_kernel<1>, grid, dim3(64, 4), 0, st, za);
-  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
-                     (long)grid.x * grid.y * grid.z, norm2_dev);
-  HIPCHK(hipGetLastError());
-  return MF_OK;
+  ProfScope ps(h, P_MULT, 12.0 * N, st);  // algorithmic: read g, write b' (FP32)
+  mf::FaceFoldArgs f;
+  f.n1 = n1; f.n2 = n2; f.n3 = n3;
+  f.Mx = rows(h->M[0]); f.My = rows(h->M[1]); f.Mz = rows(h->M[2]);
+  f.bcx = bcx; f.bcy = bcy; f.bcz = bcz;
+  f.fbx = h->fbf;
+  f.fby = f.fbx + 2L * n2 * n3;
+  f.fbz = f.fby + 2L * n1 * n3;
+  const long nf = 2 * ((long)n2 * n3 + (long)n1 * n3 + (long)n1 * n2);
+  hipLaunchKernelGGL(mf::face_fold_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, f);
+  return launch_residual<false>(h, res_args(h, nullptr, g, 0.0), norm2_dev, st);
 }
 }  // namespace
 
@@ -701,8 +755,9 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
                                  void *stream) {
   if (!h || !g || !u || !bcx || !bcy || !bcz)
     return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: null pointer");
-  if (precision != MF_FP64 && precision != MF_MIXED)
-    return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: precision must be MF_FP64 or MF_MIXED");
+  if (precision != MF_FP64 && precision != MF_MIXED && precision != MF_MIXED_BF16X3)
+    return mf::fail(MF_ERR_ARG,
+                    "mf_solver3d_solve: precision must be MF_FP64, MF_MIXED or MF_MIXED_BF16X3");
   if (g == u) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: g and u must not alias");
   if (!std::isfinite(sigma)) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: sigma is not finite");
   hipStream_t st = (hipStream_t)stream;
@@ -719,19 +774,24 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
     HIPCHK(hipGetLastError());
     return launch_faces(h, u, bcx, bcy, bcz, st);
   }
-  // ---- mixed precision: FP32 MFMA transforms, FP64 residual refinement
+  // ---- mixed precision: FP32-class MFMA transforms, FP64 residual refinement
   if (!(tol > 0.0)) tol = 1e-12;
   if (max_iter <= 0) max_iter = 8;
+  auto inner = [&](const float *in, bool accumulate) {
+    if (precision == MF_MIXED_BF16X3)
+      transforms_bf3(h, in, h->F1, h->F2, sigma, u, accumulate, st);
+    else
+      transforms<float, float>(h, in, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
+                               h->Vyt32, h->Vzt32, sigma, u, accumulate, st);
+  };
   int rc = rhs_prime(h, g, bcx, bcy, bcz, h->norms + 0, st);
   if (rc != MF_OK) return rc;
   // first low-precision solve: u_int = S(b')
-  transforms<float, double>(h, h->T2, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
-                            h->Vyt32, h->Vzt32, sigma, u, false, st);
+  inner(h->R32, false);
   int it = 1;
   double rel = 0.0;
   for (;;) {
-    HIPCHK(hipMemsetAsync(h->norms + 1, 0, sizeof(double), st));
-    rc = residual(h, u, sigma, h->T2, h->T3, h->norms + 1, st);
+    rc = residual(h, u, g, sigma, h->norms + 1, st);
     if (rc != MF_OK) return rc;
     HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -743,8 +803,7 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
       rep->iterations = it;
     }
     if (!(rel > tol) || it >= max_iter) break;
-    transforms<float, double>(h, h->T3, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
-                              h->Vyt32, h->Vzt32, sigma, u, true, st);
+    inner(h->R32, true);
     ++it;
   }
   rc = launch_faces(h, u, bcx, bcy, bcz, st);

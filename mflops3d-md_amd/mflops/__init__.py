@@ -23,10 +23,10 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import MF_FP64, MF_MIXED, MFError, SolveReport, check, load  # noqa: F401
+from ._lib import MF_FP64, MF_MIXED, MF_MIXED_BF16X3, MFError, SolveReport, check, load  # noqa: F401
 
 __all__ = [
-    "load", "MFError", "MF_FP64", "MF_MIXED", "SolveReport",
+    "load", "MFError", "MF_FP64", "MF_MIXED", "MF_MIXED_BF16X3", "SolveReport",
     "fortran_empty", "fortran_zeros", "to_device", "to_host", "is_fortran",
     "DerivativeCoefficients", "derivative", "derx", "dderx", "dery", "ddery", "derz", "dderz",
     "BoundaryCondition", "SolverCoeffs3D", "solver_3d", "der_coeffs_1d", "solver_coeffs_1d",

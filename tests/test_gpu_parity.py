@@ -8,8 +8,10 @@ the derivative algorithm; the discrete solution refined to ~1e-15 for solves):
 
 * derivatives and FP64 solves: |gpu - exact| <= max(1e-12, 2 |oracle64 - exact|)
   relative to max|exact|, i.e. 1e-12 or no worse than the FP64 oracle;
-* mixed-precision solves: relative residual of the FP64 operator <= 1e-12 and
-  |gpu - exact| <= 1e-12 relative.
+* mixed-precision solves: relative residual of the FP64 operator <= tol; at
+  tol 1e-12 |gpu - exact| <= 1e-10, at tol 1e-14 |gpu - exact| <= max(1e-12,
+  |oracle64 - exact|): FP64 refinement stalls near cond * eps64, so the mixed
+  solve is held to 1e-12 or to the FP64 reference algorithm's own error.
 Every test prints the three distances.
 """
 import numpy as np
@@ -31,13 +33,13 @@ def _mf():
     return mflops, torch
 
 
-def _check(label, got, exact, ora, tol=TOL, allow_oracle=True):
+def _check(label, got, exact, ora, tol=TOL, allow_oracle=True, ora_factor=2.0):
     ex = np.asarray(exact, dtype=np.float64)
     sc = max(np.abs(ex).max(), 1e-300)
     e_gpu = np.abs(got - ex).max() / sc
     e_ora = np.abs(ora - ex).max() / sc
     e_go = np.abs(got - ora).max() / sc
-    bar = max(tol, 2 * e_ora) if allow_oracle else tol
+    bar = max(tol, ora_factor * e_ora) if allow_oracle else tol
     print("%s: |gpu-exact| %.2e  |oracle64-exact| %.2e  |gpu-oracle64| %.2e  (bar %.1e)"
           % (label, e_gpu, e_ora, e_go, bar), flush=True)
     assert e_gpu <= bar, (label, e_gpu, e_ora)
@@ -228,23 +230,27 @@ def test_solve_fp64(shape, bct, sigma):
     assert np.all(got[0, 0, :] == 7.0) and np.all(got[:, -1, -1] == 7.0)  # edges untouched
 
 
+MIXED = {"fp32": 1, "bf16x3": 2}  # MF_MIXED, MF_MIXED_BF16X3
+
+
+@pytest.mark.parametrize("inner", sorted(MIXED))
 @pytest.mark.parametrize("shape,bct,sigma", CASES)
-def test_solve_mixed(shape, bct, sigma):
+def test_solve_mixed(shape, bct, sigma, inner):
     """Default tol 1e-12: the residual contract; tol 1e-14: FP64-level error."""
     mf, torch = _mf()
     grids, sc, g, bcs, uin = _make_problem(shape, bct, sigma, 12)
     ora, exact = _exact_full(sc, g, bcs, sigma, shape)
     exact = _faces_from(sc, exact, bcs)
     for tol, err_bar in ((1e-12, 1e-10), (1e-14, 1e-12)):
-        tag = "mixed tol=%g shape=%s bct=%s sigma=%g" % (tol, shape, bct, sigma)
+        tag = "mixed-%s tol=%g shape=%s bct=%s sigma=%g" % (inner, tol, shape, bct, sigma)
         print("run " + tag, flush=True)
-        got, rep = _solve_gpu(mf, grids, bct, g, bcs, sigma, mf.MF_MIXED, tol=tol)
+        got, rep = _solve_gpu(mf, grids, bct, g, bcs, sigma, MIXED[inner], tol=tol)
         true_res = _true_residual(sc, got, g, bcs, sigma)
-        print("  refinement: %d FP32 solves, residual %.2e (first %.2e), 80-bit residual %.2e"
+        print("  refinement: %d inner solves, residual %.2e (first %.2e), 80-bit residual %.2e"
               % (rep.iterations, rep.residual, rep.residual0, true_res), flush=True)
         assert rep.residual <= tol and rep.iterations >= 2, (rep.iterations, rep.residual)
         assert true_res <= 2 * tol
-        _check(tag, got, exact, ora, tol=err_bar, allow_oracle=False)
+        _check(tag, got, exact, ora, tol=err_bar, allow_oracle=(tol < 1e-13), ora_factor=1.0)
         assert np.all(got[0, 0, :] == 7.0) and np.all(got[:, -1, -1] == 7.0)
 
 
@@ -265,8 +271,8 @@ def test_solve_c1_helmholtz_and_poisson():
         ue = o.solve_exact(sc, g, bcx, bcy, bcz, sigma)
         exact = ora.copy()
         exact[1:-1, 1:-1, 1:-1] = np.asarray(ue, dtype=np.float64)
-        for prec in (mf.MF_FP64, mf.MF_MIXED):
-            tag = "c1 sigma=%g precision=%s" % (sigma, "fp64" if prec == 0 else "mixed")
+        for prec in (mf.MF_FP64, mf.MF_MIXED, mf.MF_MIXED_BF16X3):
+            tag = "c1 sigma=%g precision=%s" % (sigma, ("fp64", "mixed", "bf16x3")[prec])
             print("run " + tag, flush=True)
             got, rep = _solve_gpu(mf, (gx, gy, gz), bct, g, (bcx, bcy, bcz), sigma, prec,
                                   u0=np.zeros_like(u))
