@@ -118,8 +118,9 @@ int mf_solver3d_destroy(mf_solver3d *sc);
  * untouched).  precision MF_FP64: the reference algorithm in FP64.
  * MF_MIXED / MF_MIXED_BF16X3: FP32 / split-bf16 MFMA transforms with
  * FP64-residual iterative refinement until ||r||/||b|| <= tol (tol <= 0
- * selects 1e-12) or max_iter solves (max_iter <= 0 selects 8); both
- * synchronise the stream (they read back residual norms). */
+ * selects 1e-12) or max_iter solves (max_iter <= 0 selects 8, at most 32);
+ * for tol < 1e-6 at least two solves are made (one FP32-class solve leaves
+ * ~1e-7).  Both synchronise the stream (they read back residual norms). */
 int mf_solver3d_solve(mf_solver3d *sc, const double *g, double sigma, double *u,
                       const double *bcx, const double *bcy, const double *bcz,
                       int precision, double tol, int max_iter,

@@ -134,24 +134,42 @@ __global__ void __launch_bounds__(256)
         for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Mfma<T>::op(bf[ni], af[mi], acc[mi][ni]);
     }
   }
-  // epilogue: lane holds C(m0+wm+16mi+lane%16, n0+wn+16ni+row(lane,r))
+  // epilogue: lane holds C(m0+wm+16mi+lane%16, n0+wn+16ni+row(lane,r)).
+  // Index splits (m -> i1,i2; n -> j,k) take one 32-bit division per lane and
+  // then advance with wrap-around.
+  int i1 = 0, i2 = 0;
+  if (EPI == EPI_DIVIDE) {
+    const int mb = (int)(m0 + wm + (lane & 15));
+    i1 = mb % ep.n1;
+    i2 = mb / ep.n1;
+  }
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
     const long m = m0 + wm + mi * 16 + (lane & 15);
+    if (mi > 0 && EPI == EPI_DIVIDE) {
+      i1 += 16;
+      while (i1 >= ep.n1) { i1 -= ep.n1; ++i2; }
+    }
     if (m >= M) continue;
     double lxy = 0.0, big_xy = 0.0;
-    long ubase = 0;
     if (EPI == EPI_DIVIDE) {
-      const int i1 = (int)(m % ep.n1), i2 = (int)(m / ep.n1);
       lxy = ep.lx[i1] + ep.ly[i2];
       big_xy = fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2]));
     }
-    if (EPI == EPI_SCATTER || EPI == EPI_SCATTER_ADD) ubase = m + 1;
+    long ubase = 0;
+    int jb = 0, kb = 0;
+    if (EPI == EPI_SCATTER || EPI == EPI_SCATTER_ADD) {
+      ubase = m + 1;
+      const int nb = (int)(n0 + wn);
+      jb = nb % ep.n2;
+      kb = nb / ep.n2;
+    }
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long n = n0 + wn + ni * 16 + Mfma<T>::row(lane, r);
+        const int dn = ni * 16 + Mfma<T>::row(lane, r);
+        const long n = n0 + wn + dn;
         if (n >= N) continue;
         const double v = (double)acc[mi][ni][r];
         if (EPI == EPI_STORE) {
@@ -160,9 +178,13 @@ __global__ void __launch_bounds__(256)
           const double lzn = ep.lz[n];
           const double big = fmax(fmax(fabs(ep.sigma), big_xy), fabs(lzn));
           const double den = lxy + lzn + ep.sigma;
-          C[m + ldc * n] = (TC)((big < 1e-11) ? 0.0 : v / den);
+          if (sizeof(TC) == 8)
+            C[m + ldc * n] = (TC)((big < 1e-11) ? 0.0 : v / den);
+          else  // FP32 inner solve: FP32 reciprocal
+            C[m + ldc * n] = (TC)((big < 1e-11) ? 0.0f : (float)v * __builtin_amdgcn_rcpf((float)den));
         } else {
-          const long j = n % ep.n2, k = n / ep.n2;
+          int j = jb + dn, k = kb;
+          while (j >= ep.n2) { j -= ep.n2; ++k; }
           double *p = ep.u + ubase + (long)ep.nx * ((j + 1) + (long)ep.ny * (k + 1));
           if (EPI == EPI_SCATTER)
             *p = v;

@@ -33,9 +33,11 @@ __device__ __forceinline__ void split3(float x, __bf16 &p0, __bf16 &p1, __bf16 &
 
 // WLEFT: A is the eigenvector matrix (x-transform), B the field; otherwise
 // A is the field and B the eigenvector matrix (y/z transforms).
-// Wsplit: [3][WROWS][KP] bf16, zero beyond the matrix, with WROWS a multiple
-// of the tile side along the eigenvector dimension, so its tile loads need no
-// guards.  Loads are branch-free (clamped addresses, k >= K selected to 0)
+// Wsplit: bf16 [3][KP/8][WROWS][8] (part, 8-column group of k, row, k within
+// the group), zero beyond the matrix, with WROWS a multiple of the tile side
+// along the eigenvector dimension, so its tile loads need no guards; lanes
+// read consecutive rows (coalesced) and write them to LDS rows 80 bytes apart
+// (bank-conflict free).  Loads are branch-free (clamped addresses, k >= K selected to 0)
 // and the next k-step's tile is fetched into registers while the current one
 // is multiplied.
 // The block has BM*BN/4096 wavefronts, each computing a 64 x 64 tile.
@@ -79,8 +81,9 @@ __global__ void __launch_bounds__(BM * BN / 64)
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
       const int e = tid + NT * i;
-      const int c = e % (BK / 8), r = (e / (BK / 8)) % RW, p = e / ((BK / 8) * RW);
-      wreg[i] = *reinterpret_cast<const bf16x8_t *>(Wsplit + ((size_t)p * WROWS + w0 + r) * KP + kt + 8 * c);
+      const int r = e % RW, c = (e / RW) % (BK / 8), p = e / ((BK / 8) * RW);
+      wreg[i] = *reinterpret_cast<const bf16x8_t *>(
+          Wsplit + (((size_t)p * (KP / 8) + kt / 8 + c) * WROWS + w0 + r) * 8);
     }
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
@@ -100,7 +103,7 @@ __global__ void __launch_bounds__(BM * BN / 64)
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
       const int e = tid + NT * i;
-      const int c = e % (BK / 8), r = (e / (BK / 8)) % RW, p = e / ((BK / 8) * RW);
+      const int r = e % RW, c = (e / RW) % (BK / 8), p = e / ((BK / 8) * RW);
       *reinterpret_cast<bf16x8_t *>(&Ws[p][r][8 * c]) = wreg[i];
     }
 #pragma unroll
@@ -156,14 +159,35 @@ __global__ void __launch_bounds__(BM * BN / 64)
   }
   // ---- epilogue (as gemm_mode_kernel; f32 output layout).  The divide uses
   // an FP32 reciprocal of the FP64 eigenvalue sum: the result is an FP32
-  // value of an approximate inner solve.
+  // value of an approximate inner solve.  Index splits advance from one
+  // 32-bit division per lane.
+  int i1 = 0, i2 = 0;
+  if (EPI == EPI_DIVIDE) {
+    const int mb = (int)(m0 + wm + (lane & 15));
+    i1 = mb % ep.n1;
+    i2 = mb / ep.n1;
+  }
+  int jb = 0, kb = 0;
+  if (EPI == EPI_SCATTER || EPI == EPI_SCATTER_ADD) {
+    const int nb = (int)(n0 + wn + 4 * (lane >> 4));
+    jb = nb % ep.n2;
+    kb = nb / ep.n2;
+  }
+  auto uidx = [&](long m, int dn) {
+    int j = jb + dn, k = kb;
+    while (j >= ep.n2) { j -= ep.n2; ++k; }
+    return (m + 1) + (long)ep.nx * ((j + 1) + (long)ep.ny * (k + 1));
+  };
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi) {
     const long m = m0 + wm + mi * 16 + (lane & 15);
+    if (mi > 0 && EPI == EPI_DIVIDE) {
+      i1 += 16;
+      while (i1 >= ep.n1) { i1 -= ep.n1; ++i2; }
+    }
     if (m >= M) continue;
     double lxy = 0.0, big_xy = 0.0;
     if (EPI == EPI_DIVIDE) {
-      const int i1 = (int)(m % ep.n1), i2 = (int)(m / ep.n1);
       lxy = ep.lx[i1] + ep.ly[i2] + ep.sigma;
       big_xy = fmax(fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2])), fabs(ep.sigma));
     }
@@ -174,20 +198,17 @@ __global__ void __launch_bounds__(BM * BN / 64)
       for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const long n = n0 + wn + ni * 16 + 4 * (lane >> 4) + r;
-          const long nc = n < N ? n : N - 1;
-          const long j = nc % ep.n2, k = nc / ep.n2;
-          old[ni][r] = ep.u[(m + 1) + (long)ep.nx * ((j + 1) + (long)ep.ny * (k + 1))];
+          const int dn = ni * 16 + r;
+          const bool ok = n0 + wn + 4 * (lane >> 4) + dn < N;
+          old[ni][r] = ok ? ep.u[uidx(m, dn)] : 0.0;
         }
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const long n = n0 + wn + ni * 16 + 4 * (lane >> 4) + r;
-          if (n >= N) continue;
-          const long j = n % ep.n2, k = n / ep.n2;
-          ep.u[(m + 1) + (long)ep.nx * ((j + 1) + (long)ep.ny * (k + 1))] =
-              old[ni][r] + (double)acc[mi][ni][r];
+          const int dn = ni * 16 + r;
+          if (n0 + wn + 4 * (lane >> 4) + dn >= N) continue;
+          ep.u[uidx(m, dn)] = old[ni][r] + (double)acc[mi][ni][r];
         }
       continue;
     }
@@ -195,7 +216,8 @@ __global__ void __launch_bounds__(BM * BN / 64)
     for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long n = n0 + wn + ni * 16 + 4 * (lane >> 4) + r;
+        const int dn = ni * 16 + r;
+        const long n = n0 + wn + 4 * (lane >> 4) + dn;
         if (n >= N) continue;
         const float v = acc[mi][ni][r];
         if (EPI == EPI_STORE) {
@@ -206,8 +228,7 @@ __global__ void __launch_bounds__(BM * BN / 64)
           const float inv = __builtin_amdgcn_rcpf((float)(lxy + lzn));
           C[m + ldc * n] = (TC)((big < 1e-11) ? 0.0f : v * inv);
         } else {
-          const long j = n % ep.n2, k = n / ep.n2;
-          ep.u[(m + 1) + (long)ep.nx * ((j + 1) + (long)ep.ny * (k + 1))] = (double)v;
+          ep.u[uidx(m, dn)] = (double)v;
         }
       }
   }

@@ -20,12 +20,13 @@
 // marches over the planes the chunk's z-windows reach (RZ_CZ + 4 inside the
 // domain).  Each plane of u and g is staged with a 4-point halo in x and y,
 // 64 x 24 points: one wavefront spans the staged width, so wave w computes
-// the y-stencils of output rows 4w..4w+3 (rows uniform per wave: weights
-// come from the scalar cache, and inside the domain the four rows reuse one
-// 8-row register segment) and then the x-stencils of the same rows, lane =
-// column.  Each plane's A and G are added into the chunk's rows in registers.
-// HBM traffic: u and g read once (halo re-reads hit L2), r' written once in
-// FP32 (the inner solve's input).
+// the y-stencils of output rows 4w..4w+3 (rows uniform per wave: inside the
+// domain the four rows reuse one 8-row register segment) and then the
+// x-stencils of the same rows, lane = column.  Each plane's A and G are added
+// into the chunk's rows in registers; weights come from per-block LDS tables
+// and planes are fetched two ahead of their use.  HBM traffic: u and g read
+// once (halo re-reads hit L2), r' written once in FP32 (the inner solve's
+// input).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -38,7 +39,8 @@ struct BandRows {      // one banded 1-D operator: row i acts on c0[i]..c0[i]+4
 
 constexpr int BAND_W = 5;  // window width of every banded row
 constexpr int RZ_H = 4;    // halo: the farthest reach of a row
-constexpr int RZ_TX = 64 - 2 * RZ_H, RZ_TY = 16, RZ_CZ = 8;
+constexpr int RZ_NW = 4, RZ_RPW = 4;  // wavefronts per block, tile rows per wavefront
+constexpr int RZ_TX = 64 - 2 * RZ_H, RZ_TY = RZ_NW * RZ_RPW, RZ_CZ = 8;
 
 struct ResArgs {
   int n1, n2, n3;
@@ -54,14 +56,19 @@ struct ResArgs {
 
 // RES = false: right-hand side b' = P g_f (u is not read);
 // RES = true:  residual r' = P g_f - B u.
-// block (64,4), grid (ceil(n1/RZ_TX), ceil(n2/RZ_TY), ceil(n3/RZ_CZ))
+// block (64, RZ_NW), grid: one block per tile (ceil(n1/RZ_TX) ceil(n2/RZ_TY)
+// ceil(n3/RZ_CZ) in all, any shape); consecutive block ids go round-robin
+// to the 8 XCDs, so the tile order is remapped to give each XCD a contiguous
+// run of neighbouring tiles (their halos then hit the XCD's own L2).
 template <bool RES>
-__global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
+__global__ void __launch_bounds__(64 * RZ_NW) residual_kernel(ResArgs a) {
+  constexpr int NT = 64 * RZ_NW;
   constexpr int SW = RZ_TX + 2 * RZ_H, SH = RZ_TY + 2 * RZ_H;  // staged tile 64 x 24
   constexpr int SP = SW + 1;                                   // row pitch
-  constexpr int NS = SW * SH / 256;                            // staged points per thread
+  constexpr int NS = SW * SH / NT;                             // staged points per thread
+  constexpr int SEG = RZ_RPW + BAND_W - 1;                     // centred y-window rows
   constexpr int NP = RZ_CZ + 2 * RZ_H;                         // most planes a chunk reaches
-  static_assert(SW == 64 && NS * 256 == SW * SH, "one wavefront per staged row");
+  static_assert(SW == 64 && NS * NT == SW * SH, "one wavefront per staged row");
   __shared__ double U[SH][SP], H[SH][SP];
   __shared__ double T1[RZ_TY][SP], T2[RZ_TY][SP], T3[RZ_TY][SP];
   __shared__ double WY[RZ_TY][2][BAND_W];       // My, Ry rows of the tile
@@ -70,7 +77,16 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
   const int lane = threadIdx.x, tid = lane + 64 * threadIdx.y;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);  // wave index (uniform)
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
-  const int i0 = blockIdx.x * RZ_TX, j0 = blockIdx.y * RZ_TY, z0 = blockIdx.z * RZ_CZ;
+  const int nbx = (n1 + RZ_TX - 1) / RZ_TX, nby = (n2 + RZ_TY - 1) / RZ_TY;
+  int i0, j0, z0;
+  {
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int q = nb / 8, r = nb % 8, x = bid % 8, l = bid / 8;
+    const int t = (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+    i0 = (t % nbx) * RZ_TX;
+    j0 = ((t / nbx) % nby) * RZ_TY;
+    z0 = (t / (nbx * nby)) * RZ_CZ;
+  }
   // planes reached by the chunk's rows (window starts are non-decreasing)
   const int zl = min(z0 + RZ_CZ, n3) - 1;
   int p_lo = a.Mz.c0[z0], p_hi = a.Mz.c0[zl] + BAND_W - 1;
@@ -86,7 +102,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
     WY[r][q][m] = (q == 0 || RES) ? B.w[8 * j + m] : 0.0;
     if (m == 0) CY[r][q] = B.c0[j] - (j0 - RZ_H);
   }
-  for (int e = tid; e < NP * RZ_CZ; e += 256) {
+  for (int e = tid; e < NP * RZ_CZ; e += NT) {
     const int sp = e / RZ_CZ, kk = e % RZ_CZ;
     const int p = p_lo + sp, k = min(z0 + kk, n3 - 1);
     const int dm = p - a.Mz.c0[k], dr = p - a.Rz.c0[k];
@@ -103,20 +119,22 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
   bool sok[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const int sj = j0 - RZ_H + w + 4 * s;
+    const int sj = j0 - RZ_H + w + RZ_NW * s;
     sok[s] = iok && sj >= 0 && sj < n2;
     soff[s] = sic + a.ldj * min(max(sj, 0), n2 - 1);
   }
-  double ru[NS], rg[NS];
-  auto fetch = [&](int p) {
+  // two register sets: planes are fetched two ahead of their use
+  double ru[2][NS], rg[2][NS];
+  auto fetch = [&](int p, int b) {
     const long po = a.ldk * p;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      rg[s] = a.g[soff[s] + po];
-      if (RES) ru[s] = a.u[soff[s] + po];
+      rg[b][s] = a.g[soff[s] + po];
+      if (RES) ru[b][s] = a.u[soff[s] + po];
     }
   };
-  fetch(p_lo);
+  fetch(p_lo, 0);
+  if (p_lo + 1 <= p_hi) fetch(p_lo + 1, 1);
   // x: lane = staged column; outputs for lanes RZ_H .. RZ_H+RZ_TX-1
   const int i = i0 - RZ_H + lane;
   const bool xout = lane >= RZ_H && lane < RZ_H + RZ_TX && i < n1;
@@ -130,42 +148,44 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
     wmx[m] = a.Mx.w[8 * ic + m];
     wrx[m] = RES ? a.Rx.w[8 * ic + m] : 0.0;
   }
-  double acc[RZ_CZ][4];
+  double acc[RZ_CZ][RZ_RPW];
 #pragma unroll
   for (int kk = 0; kk < RZ_CZ; ++kk)
 #pragma unroll
-    for (int d = 0; d < 4; ++d) acc[kk][d] = 0.0;
+    for (int d = 0; d < RZ_RPW; ++d) acc[kk][d] = 0.0;
   __syncthreads();
-  // y rows of this wave: output rows 4w..4w+3; centred windows use staged
-  // rows 4w+d+2 .. 4w+d+6, one 8-row segment
-  const int jr0 = j0 + 4 * w;
-  bool yfast = jr0 + 3 < n2;
+  // y rows of this wave: output rows R w .. R w + R-1 (R = RZ_RPW); centred
+  // windows use staged rows R w + d + 2 .. R w + d + 6, one SEG-row segment
+  const int jr0 = j0 + RZ_RPW * w;
+  bool yfast = jr0 + RZ_RPW - 1 < n2;
 #pragma unroll
-  for (int d = 0; d < 4; ++d)
-    yfast = yfast && CY[4 * w + d][0] == 4 * w + d + 2 && (!RES || CY[4 * w + d][1] == 4 * w + d + 2);
+  for (int d = 0; d < RZ_RPW; ++d) {
+    const int r = RZ_RPW * w + d;
+    yfast = yfast && CY[r][0] == r + 2 && (!RES || CY[r][1] == r + 2);
+  }
 
-  for (int p = p_lo; p <= p_hi; ++p) {
+  auto plane = [&](int p, int b) {
     // 1. stage plane p: U = u, H = g - sigma u
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double uv = (RES && sok[s]) ? ru[s] : 0.0;
-      const double gv = sok[s] ? rg[s] : 0.0;
-      U[w + 4 * s][lane] = uv;
-      H[w + 4 * s][lane] = RES ? fma(-a.sigma, uv, gv) : gv;
+      const double uv = (RES && sok[s]) ? ru[b][s] : 0.0;
+      const double gv = sok[s] ? rg[b][s] : 0.0;
+      U[w + RZ_NW * s][lane] = uv;
+      H[w + RZ_NW * s][lane] = RES ? fma(-a.sigma, uv, gv) : gv;
     }
     __syncthreads();
-    if (p < p_hi) fetch(p + 1);
-    // 2. y-stencils of rows jr0..jr0+3, column = lane
+    if (p + 2 <= p_hi) fetch(p + 2, b);
+    // 2. y-stencils of the wave's rows, column = lane
     if (yfast) {
-      double su[8], sh[8];
+      double su[SEG], sh[SEG];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        sh[t] = H[4 * w + 2 + t][lane];
-        if (RES) su[t] = U[4 * w + 2 + t][lane];
+      for (int t = 0; t < SEG; ++t) {
+        sh[t] = H[RZ_RPW * w + 2 + t][lane];
+        if (RES) su[t] = U[RZ_RPW * w + 2 + t][lane];
       }
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const double *wm = WY[4 * w + d][0], *wr = WY[4 * w + d][1];
+      for (int d = 0; d < RZ_RPW; ++d) {
+        const double *wm = WY[RZ_RPW * w + d][0], *wr = WY[RZ_RPW * w + d][1];
         double t1 = 0.0, t2 = 0.0, t3 = 0.0;
 #pragma unroll
         for (int m = 0; m < BAND_W; ++m) {
@@ -175,16 +195,16 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
             t2 = fma(wr[m], su[d + m], t2);
           }
         }
-        T3[4 * w + d][lane] = t3;
+        T3[RZ_RPW * w + d][lane] = t3;
         if (RES) {
-          T1[4 * w + d][lane] = t1;
-          T2[4 * w + d][lane] = t2;
+          T1[RZ_RPW * w + d][lane] = t1;
+          T2[RZ_RPW * w + d][lane] = t2;
         }
       }
     } else {
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const int r = 4 * w + d;
+      for (int d = 0; d < RZ_RPW; ++d) {
+        const int r = RZ_RPW * w + d;
         const int cm = CY[r][0], cr = CY[r][1];
         double t1 = 0.0, t2 = 0.0, t3 = 0.0;
 #pragma unroll
@@ -203,11 +223,11 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
       }
     }
     __syncthreads();
-    // 3. x-stencils of rows 4w..4w+3 at column `lane`, then into the chunk's rows
+    // 3. x-stencils of the wave's rows at column `lane`, then into the chunk's rows
     const int sp = p - p_lo;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      const int r = 4 * w + d;
+    for (int d = 0; d < RZ_RPW; ++d) {
+      const int r = RZ_RPW * w + d;
       double A = 0.0, G = 0.0;
 #pragma unroll
       for (int m = 0; m < BAND_W; ++m) {
@@ -225,6 +245,11 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
         if (RES) acc[kk][d] = fma(-wz.y, A, acc[kk][d]);
       }
     }
+    // (the next plane's first barrier orders these T reads before its writes)
+  };
+  for (int p = p_lo; p <= p_hi; p += 2) {
+    plane(p, 0);
+    if (p + 1 <= p_hi) plane(p + 1, 1);
   }
   // 4. boundary fold, store r' (FP32), norm partials
   double contrib = 0.0;
@@ -232,7 +257,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
     const double ex1 = a.ex[i], ex2 = a.ex[n1 + i];
     const long sx = (long)n2 * n3, sy = (long)n1 * n3, sz = (long)n1 * n2;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
+    for (int d = 0; d < RZ_RPW; ++d) {
       const int j = jr0 + d;
       if (j >= n2) continue;
       const double ey1 = a.ey[j], ey2 = a.ey[n2 + j];
@@ -253,13 +278,15 @@ __global__ void __launch_bounds__(256) residual_kernel(ResArgs a) {
       }
     }
   }
-  __shared__ double wsum[4];
+  __shared__ double wsum[RZ_NW];
   for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
   if (lane == 0) wsum[w] = contrib;
   __syncthreads();
-  if (tid == 0)
-    a.part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
-        (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < RZ_NW; ++q) t += wsum[q];
+    a.part[blockIdx.x] = t;
+  }
 }
 
 // Face products of the boundary fold: for the x faces F(j,k,s) =

@@ -145,7 +145,7 @@ struct mf_solver3d {
   double *Wx = nullptr, *Vx = nullptr, *Wyt = nullptr, *Vyt = nullptr, *Wzt = nullptr, *Vzt = nullptr;
   float *Wx32 = nullptr, *Vx32 = nullptr, *Wyt32 = nullptr, *Vyt32 = nullptr, *Wzt32 = nullptr,
         *Vzt32 = nullptr;  // mixed: W' = inv(V) inv(lhs)
-  // the same six matrices split into three bf16 parts, [part][row][k]
+  // the same six matrices split into three bf16 parts, [part][k/8][row][k%8]
   // (row = the eigenvector-dimension index, zero-padded to wrows[axis] rows
   // and kp[axis] columns)
   __bf16 *Wsp[3] = {nullptr, nullptr, nullptr}, *Vsp[3] = {nullptr, nullptr, nullptr};
@@ -190,6 +190,7 @@ namespace {
   } while (0)
 
 enum { P_FOLD = 0, P_T64 = 1, P_T32 = 2, P_RES = 3, P_MULT = 4, P_FACES = 5 };
+constexpr int MF_MAX_REFINE = 32;  // cap on max_iter (residual norms kept per solve)
 
 // Brackets the launches made during its lifetime with a pair of events.
 struct ProfScope {
@@ -451,7 +452,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->fold = take(fold.size());
     h->neum = take(neum.size());
     h->bandw = take(bandw.size());
-    h->norms = take(8);
+    h->norms = take(MF_MAX_REFINE + 1);
     h->part = take(nparts);
     h->T1 = take(N); h->T2 = take(N);
     h->F1 = (float *)take((N + 1) / 2);
@@ -495,7 +496,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     if (e == hipSuccess) e = upf(fW[a], Wp[a]);
     if (e == hipSuccess) e = upf(fV[a], V[a]);
   }
-  // three-part bf16 splits: [part][row][k]; the x matrices are the left
+  // three-part bf16 splits: [part][k/8][row][k%8]; the x matrices are the left
   // operand (row = m of W(m,k)), the stored y/z transposes the right one
   // (row = n of B(k,n))
   for (int a = 0; a < 3 && e == hipSuccess; ++a) {
@@ -508,7 +509,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
           const double v = (a == 0) ? src[r + (size_t)m * k] : src[k + (size_t)m * r];
           uint16_t p[3];
           bf16_split3((float)v, p);
-          for (int q = 0; q < 3; ++q) sp[((size_t)q * wr + r) * kp + k] = p[q];
+          for (int q = 0; q < 3; ++q)
+            sp[(((size_t)q * (kp / 8) + k / 8) * wr + r) * 8 + k % 8] = p[q];
         }
       e = up(which == 0 ? (void *)h->Wsp[a] : (void *)h->Vsp[a], sp.data(), sp.size() * 2);
     }
@@ -520,7 +522,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
   if (e == hipSuccess) e = up(h->bandw, bandw.data(), bandw.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->bandc, bandc.data(), bandc.size() * sizeof(int));
   if (e == hipSuccess) e = up(h->bctd, h->bct, 6 * sizeof(int));
-  if (e == hipSuccess) e = hipHostMalloc((void **)&h->hnorms, 8 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&h->hnorms, (MF_MAX_REFINE + 1) * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
     mf_solver3d_destroy(h);
     return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_create: upload: ") + hipGetErrorString(e));
@@ -711,11 +714,12 @@ mf::ResArgs res_args(mf_solver3d *h, const double *u, const double *g, double si
 
 template <bool RES>
 int launch_residual(mf_solver3d *h, const mf::ResArgs &a, double *norm2_dev, hipStream_t st) {
-  const dim3 grid((h->n1 + mf::RZ_TX - 1) / mf::RZ_TX, (h->n2 + mf::RZ_TY - 1) / mf::RZ_TY,
-                  (h->n3 + mf::RZ_CZ - 1) / mf::RZ_CZ);
-  hipLaunchKernelGGL(mf::residual_kernel<RES>, grid, dim3(64, 4), 0, st, a);
+  const unsigned nb = (unsigned)(((h->n1 + mf::RZ_TX - 1) / mf::RZ_TX) *
+                                 ((h->n2 + mf::RZ_TY - 1) / mf::RZ_TY) *
+                                 ((h->n3 + mf::RZ_CZ - 1) / mf::RZ_CZ));
+  hipLaunchKernelGGL(mf::residual_kernel<RES>, dim3(nb), dim3(64, mf::RZ_NW), 0, st, a);
   hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
-                     (long)grid.x * grid.y * grid.z, norm2_dev);
+                     (long)nb, norm2_dev);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
@@ -784,27 +788,40 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
       transforms<float, float>(h, in, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
                                h->Vyt32, h->Vzt32, sigma, u, accumulate, st);
   };
+  if (max_iter > MF_MAX_REFINE) max_iter = MF_MAX_REFINE;
   int rc = rhs_prime(h, g, bcx, bcy, bcz, h->norms + 0, st);
   if (rc != MF_OK) return rc;
-  // first low-precision solve: u_int = S(b')
+  // first low-precision solve: u_int = S(b'); norms[it] = ||r'||^2 after it solves
   inner(h->R32, false);
+  rc = residual(h, u, g, sigma, h->norms + 1, st);
+  if (rc != MF_OK) return rc;
   int it = 1;
+  // One FP32-class solve leaves a residual near 1e-7, so below that
+  // tolerance the first correction is enqueued without waiting for the
+  // first residual (one host round trip less).
+  if (tol < 1e-6 && max_iter >= 2) {
+    inner(h->R32, true);
+    rc = residual(h, u, g, sigma, h->norms + 2, st);
+    if (rc != MF_OK) return rc;
+    it = 2;
+  }
   double rel = 0.0;
   for (;;) {
-    rc = residual(h, u, g, sigma, h->norms + 1, st);
-    if (rc != MF_OK) return rc;
-    HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, (it + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const double b2 = h->hnorms[0], r2 = h->hnorms[1];
-    rel = (b2 > 0.0) ? std::sqrt(r2 / b2) : std::sqrt(r2);
+    const double b2 = h->hnorms[0];
+    auto relres = [&](double r2) { return (b2 > 0.0) ? std::sqrt(r2 / b2) : std::sqrt(r2); };
+    rel = relres(h->hnorms[it]);
     if (rep) {
-      if (it == 1) rep->residual0 = rel;
+      rep->residual0 = relres(h->hnorms[1]);
       rep->residual = rel;
       rep->iterations = it;
     }
     if (!(rel > tol) || it >= max_iter) break;
     inner(h->R32, true);
     ++it;
+    rc = residual(h, u, g, sigma, h->norms + it, st);
+    if (rc != MF_OK) return rc;
   }
   rc = launch_faces(h, u, bcx, bcy, bcz, st);
   if (rc != MF_OK) return rc;
