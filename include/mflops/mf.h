@@ -164,6 +164,49 @@ int mf_solver3d_profile_read(mf_solver3d *sc, double ms[MF_PROF_CLASSES],
                              int count[MF_PROF_CLASSES], double work[MF_PROF_CLASSES],
                              int reset);
 
+/* ---------------------------------------------------------------------
+ * Prediction-projection Navier-Stokes step (navier3d.f90:48-286,
+ * class_navier_3D.f90), single domain, on the device.  The reference's
+ * options are nlt (nonlinear form: 2 skew-symmetric, 1 advective), tou (BDF
+ * order 1..3) and top (pressure extrapolation order top-1, 1..3); pt=1
+ * (non-incremental rotational correction with the tangential boundary
+ * gradient, add_boundary_gradient :695-791), no mapping, one sub-iteration.
+ * Velocity is Dirichlet and phi Neumann on all six faces
+ * (navier_initialization :2166-2169).  Fields are full (nx,ny,nz) device
+ * arrays, bc arrays as for the solver.
+ * --------------------------------------------------------------------- */
+typedef struct mf_ns mf_ns;
+
+/* <- navier_initialization (class_navier_3D.f90:2084-2345): grids, Re, dt,
+ *    BDF factors, derivative and solver coefficients; fields zero. */
+int mf_ns_create(const double *gx, int nx, const double *gy, int ny, const double *gz, int nz,
+                 double re, double dt, int tou, int top, int nlt, int so, mf_ns **out);
+int mf_ns_destroy(mf_ns *ns);
+
+/* Velocity boundary values of component c (0,1,2 = u,v,w), kept until set
+ * again (<- navier_bc_velocity :395-433, whose values come from sol()). */
+int mf_ns_set_bc(mf_ns *ns, int c, const double *bcx, const double *bcy, const double *bcz,
+                 void *stream);
+/* Constant body force of component c (full field), or NULL for none
+ * (<- the sol()-based part of f(nav,var) :1043-1070). */
+int mf_ns_set_forcing(mf_ns *ns, int c, const double *f, void *stream);
+/* Constant-flow-rate force on u (<- f(nav,'rhsu') :1124-1164): uniform g
+ * with g = g0 at the first step (g0 <= 0: the reference's 3 * flow rate),
+ * then g -= (Q - Q0) fac1, Q the weighted volume mean of extrapolated u. */
+int mf_ns_set_flow_rate(mf_ns *ns, int enable, double g0, void *stream);
+/* All time levels of u, v, w := the given fields (edges zeroed); p = phi = 0. */
+int mf_ns_set_state(mf_ns *ns, const double *u, const double *v, const double *w, void *stream);
+/* nsteps time steps.  The three Helmholtz velocity solves use `precision`
+ * (MF_FP64, MF_MIXED or MF_MIXED_BF16X3, refined to tol); the all-Neumann
+ * pressure Poisson solve is always FP64 (its null mode makes the right-hand
+ * side's incompatible part an irreducible residual). */
+int mf_ns_step(mf_ns *ns, int nsteps, int precision, double tol, void *stream);
+/* Latest time level of u, v, w, p, phi into device arrays (any may be NULL). */
+int mf_ns_get(const mf_ns *ns, double *u, double *v, double *w, double *p, double *phi,
+              void *stream);
+/* Simulated time and the current uniform flow-rate force (synchronises). */
+int mf_ns_info(const mf_ns *ns, double *time, double *flow_force, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

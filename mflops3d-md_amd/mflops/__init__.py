@@ -351,3 +351,70 @@ def split_faces(row: np.ndarray, shape):
     bcy = row[a:b].reshape((n1, n3, 2), order="F")
     bcz = row[b:].reshape((n1, n2, 2), order="F")
     return bcx, bcy, bcz
+
+
+# ---------------------------------------------------------------------------
+# Navier-Stokes prediction-projection step
+# ---------------------------------------------------------------------------
+
+
+class NavierStokes:
+    """navier3d state on the device (class_navier_3D.f90:26-76) with the
+    reference's pt=1 step; see mf_ns_* in include/mflops/mf.h.
+
+    Fields are full (nx, ny, nz) Fortran-order float64 device tensors."""
+
+    def __init__(self, gx, gy, gz, re: float, dt: float, tou: int = 2, top: int = 2,
+                 nlt: int = 2, so: int = 6):
+        lib = load()
+        self._lib = lib
+        gs = [np.ascontiguousarray(g, dtype=np.float64) for g in (gx, gy, gz)]
+        self._grids = gs
+        self.shape = tuple(len(g) for g in gs)
+        h = C.c_void_p()
+        check(lib.mf_ns_create(_dptr(gs[0]), len(gs[0]), _dptr(gs[1]), len(gs[1]),
+                               _dptr(gs[2]), len(gs[2]), float(re), float(dt), int(tou),
+                               int(top), int(nlt), int(so), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mf_ns_destroy(h)
+            self._h = None
+
+    def set_bc(self, comp: int, bc: "BoundaryCondition", stream=None):
+        check(self._lib.mf_ns_set_bc(self._h, int(comp), C.c_void_p(_ptr(bc.bcx)),
+                                     C.c_void_p(_ptr(bc.bcy)), C.c_void_p(_ptr(bc.bcz)),
+                                     _stream(stream)))
+
+    def set_forcing(self, comp: int, f=None, stream=None):
+        p = C.c_void_p(_ptr(f)) if f is not None else C.c_void_p()
+        check(self._lib.mf_ns_set_forcing(self._h, int(comp), p, _stream(stream)))
+
+    def set_flow_rate(self, enable: bool = True, g0: float = 0.0, stream=None):
+        check(self._lib.mf_ns_set_flow_rate(self._h, int(bool(enable)), float(g0),
+                                            _stream(stream)))
+
+    def set_state(self, u, v, w, stream=None):
+        for t in (u, v, w):
+            if tuple(t.shape) != self.shape:
+                raise ValueError("field shape %s != %s" % (tuple(t.shape), self.shape))
+        check(self._lib.mf_ns_set_state(self._h, C.c_void_p(_ptr(u)), C.c_void_p(_ptr(v)),
+                                        C.c_void_p(_ptr(w)), _stream(stream)))
+
+    def step(self, nsteps: int = 1, precision: int = MF_FP64, tol: float = 1e-12, stream=None):
+        check(self._lib.mf_ns_step(self._h, int(nsteps), int(precision), float(tol),
+                                   _stream(stream)))
+
+    def get(self, stream=None):
+        """(u, v, w, p, phi) of the latest time level, new device tensors."""
+        out = [fortran_empty(self.shape) for _ in range(5)]
+        check(self._lib.mf_ns_get(self._h, *[C.c_void_p(_ptr(t)) for t in out],
+                                  _stream(stream)))
+        return tuple(out)
+
+    def info(self, stream=None):
+        t, g = C.c_double(), C.c_double()
+        check(self._lib.mf_ns_info(self._h, C.byref(t), C.byref(g), _stream(stream)))
+        return t.value, g.value

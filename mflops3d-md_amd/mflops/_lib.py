@@ -42,6 +42,16 @@ SIGNATURES = {
                                         C.c_int, _vp, _vp]),
     "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
     "mf_solver3d_profile_read": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "mf_ns_create": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_double, C.c_double,
+                               C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "mf_ns_destroy": (C.c_int, [_vp]),
+    "mf_ns_set_bc": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "mf_ns_set_forcing": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "mf_ns_set_flow_rate": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    "mf_ns_set_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mf_ns_step": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _vp]),
+    "mf_ns_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mf_ns_info": (C.c_int, [_vp, _dp, _dp, _vp]),
 }
 
 PROF_CLASSES = ("fold", "transform_fp64", "transform_fp32", "residual_fp64",

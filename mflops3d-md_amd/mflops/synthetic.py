@@ -105,3 +105,62 @@ def helmholtz_256_problem(n: int = 256, stretch: float = 2.0, seed: int = 2, mod
         g += amp * fx[:, None, None] * fy[None, :, None] * fz[None, None, :]
     g += noise * normal(seed, n ** 3, offset=8 * modes).reshape((n, n, n), order="F")
     return gx, gy, gz, np.asfortranarray(g)
+
+
+def channel_problem(nx: int = 192, ny: int = 256, nz: int = 192, stretch: float = 2.5,
+                    re: float = 3000.0, amp: float = 0.1, kmax: int = 8, modes: int = 8,
+                    seed: int = 4, lx: float = 2.0 * np.pi, lz: float = np.pi):
+    """Wall-bounded channel: x in [0,lx] and z in [0,lz] uniform, y in [-1,1]
+    tanh-stretched (walls at y = +-1, Re on the half-height).
+
+    Velocity = laminar parabola (1 - y^2, 0, 0) + a divergence-free
+    perturbation u' = curl(psi), psi_c = E(x,y,z) sum_m a cos(k.x + phase),
+    with E = sin^2(pi x/lx) (1-y^2)^2 sin^2(pi z/lz) so that u' and its
+    potential vanish on every face; wavenumbers 1..kmax per direction,
+    phases and amplitudes from the seeded SplitMix64 stream, scaled to
+    max|u'| = amp.  Boundary values: the parabola on every face (0 on the
+    walls).  Returns (gx, gy, gz, u, v, w, bc) with bc = {c: (bcx, bcy, bcz)}
+    and fields Fortran-ordered (nx, ny, nz).
+    """
+    gx = grid_uniform(nx, 0.0, lx)
+    gy = grid_tanh(ny, -1.0, 1.0, stretch)
+    gz = grid_uniform(nz, 0.0, lz)
+    X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+    ex, dex = np.sin(np.pi * X / lx) ** 2, np.pi / lx * np.sin(2.0 * np.pi * X / lx)
+    ey, dey = (1.0 - Y * Y) ** 2, -4.0 * Y * (1.0 - Y * Y)
+    ez, dez = np.sin(np.pi * Z / lz) ** 2, np.pi / lz * np.sin(2.0 * np.pi * Z / lz)
+    E = ex * ey * ez
+    dE = (dex * ey * ez, ex * dey * ez, ex * ey * dez)
+    r = uniform01(seed, 3 * modes * 7)
+    grad = []  # grad[c][a] = d psi_c / d x_a
+    for c in range(3):
+        S = np.zeros_like(X)
+        dS = [np.zeros_like(X) for _ in range(3)]
+        for m in range(modes):
+            q = r[7 * (c * modes + m):7 * (c * modes + m) + 7]
+            n = 1 + np.floor(kmax * q[:3]).astype(int)
+            k = (2.0 * np.pi * n[0] / lx, np.pi * n[1] / 2.0, 2.0 * np.pi * n[2] / lz)
+            a = q[6] - 0.5
+            arg = k[0] * X + k[1] * Y + k[2] * Z + 2.0 * np.pi * q[3]
+            S += a * np.cos(arg)
+            s = -a * np.sin(arg)
+            for ax in range(3):
+                dS[ax] += k[ax] * s
+        grad.append([dE[ax] * S + E * dS[ax] for ax in range(3)])
+    up = grad[2][1] - grad[1][2]
+    vp = grad[0][2] - grad[2][0]
+    wp = grad[1][0] - grad[0][1]
+    scale = amp / max(np.abs(up).max(), np.abs(vp).max(), np.abs(wp).max())
+    U = 1.0 - Y * Y
+    u = np.asfortranarray(U + scale * up)
+    v = np.asfortranarray(scale * vp)
+    w = np.asfortranarray(scale * wp)
+
+    def faces(f):
+        return (np.asfortranarray(np.stack([f[0, 1:-1, 1:-1], f[-1, 1:-1, 1:-1]], -1)),
+                np.asfortranarray(np.stack([f[1:-1, 0, 1:-1], f[1:-1, -1, 1:-1]], -1)),
+                np.asfortranarray(np.stack([f[1:-1, 1:-1, 0], f[1:-1, 1:-1, -1]], -1)))
+
+    zero = np.zeros_like(U)
+    bc = {0: faces(U), 1: faces(zero), 2: faces(zero)}
+    return gx, gy, gz, u, v, w, bc

@@ -1,0 +1,254 @@
+"""Navier-Stokes prediction-projection step: the oracle pinned by closed forms
+and invariants (CPU), then the device step against the oracle (GPU).
+
+Oracle pins (each fails for a plausible restatement mistake):
+* BDF factors differentiate t^k exactly for k <= tou, and the extrapolation of
+  order k reproduces polynomials of degree k-1 (wrong factor or index fails);
+* the nonlinear operator equals the analytic advective (nlt=1) and
+  skew-symmetric (nlt=2) forms on smooth non-solenoidal fields, to the
+  compact schemes' accuracy (a dropped 1/2 or product term fails);
+* integral weights equal a direct loop over the reference's four loops;
+* the laminar channel (parabola, body force 2/Re, or the flow-rate force)
+  is a discrete steady state to 1e-12 (sign of the viscous term, sigma,
+  Re scaling, BDF factors, pressure coupling);
+* a 3-D manufactured steady flow whose advection is not a gradient is kept
+  to the scheme's splitting error, which falls with the grid, while the same
+  run with the advective forcing's sign flipped is off by O(1);
+* the projection removes most of the divergence of a perturbed field.
+"""
+import numpy as np
+import pytest
+
+import mf_oracle as o
+import ns_oracle as ns
+
+sp = pytest.importorskip("sympy")
+
+
+# ------------------------------------------------------------------ oracle --
+
+@pytest.mark.parametrize("tou", [1, 2, 3])
+def test_bdf_factors_and_extrapolation(tou):
+    dt = 0.01
+    fac = ns.bdf_factors(tou, dt)
+    t = np.array([1.0, 1.0 - dt, 1.0 - 2 * dt, 1.0 - 3 * dt])  # t^{n+1}, t^n, t^{n-1}, t^{n-2}
+    for k in range(tou + 1):  # d/dt t^k at t = 1 is k
+        d = sum(fac[m] * t[m] ** k for m in range(4))
+        assert abs(d - k) < 1e-9 * max(1, k)
+    # and BDF-tou is not exact one degree higher
+    k = tou + 1
+    assert abs(sum(fac[m] * t[m] ** k for m in range(4)) - k) > 1e-6
+    # extrapolation of order tou to t^{n+1} from levels it(nt), it(nt-1), it(nt-2)
+    it = [3, 0, 1, 2]  # it(1) = 3 (new), it(nt) = 2, it(nt-1) = 1, it(nt-2) = 0
+    for deg in range(tou):
+        levels = [np.full((3, 3, 3), 0.0) for _ in range(4)]
+        levels[2][:] = (1.0 - dt) ** deg
+        levels[1][:] = (1.0 - 2 * dt) ** deg
+        levels[0][:] = (1.0 - 3 * dt) ** deg
+        e = ns.extrapol(levels, it, tou)
+        assert abs(e[1, 1, 1] - 1.0) < 1e-12
+    assert ns.extrapol([np.ones((3, 3, 3))] * 4, it, 0).max() == 0.0
+
+
+def _fields(n=24):
+    g = o.grid_uniform(n, 0.0, 1.0)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x, y, z = sp.symbols("x y z")
+    U = [sp.sin(x) * sp.cos(2 * y) + z ** 2, sp.exp(x / 2) * sp.sin(z) + y * x,
+         sp.cos(x + y) * (1 + z) + sp.sin(3 * y)]   # not divergence-free
+    ev = lambda e: np.asfortranarray(np.broadcast_to(
+        sp.lambdify((x, y, z), e, "numpy")(X, Y, Z), X.shape).astype(float))
+    return g, (x, y, z), U, ev
+
+
+@pytest.mark.parametrize("nlt", [1, 2])
+def test_nonlinear_operator(nlt):
+    g, X, U, ev = _fields()
+    s = ns.NSOracle(g, g, g, 1.0, 0.01, nlt=nlt)
+    s.set_state(*[ev(e) for e in U])
+    inner = (slice(3, -3),) * 3
+    for c, name in enumerate("uvw"):
+        got = s.nonlinear(0, name)
+        q = U[c]
+        adv = sum(U[a] * sp.diff(q, X[a]) for a in range(3))
+        cons = sum(sp.diff(q * U[a], X[a]) for a in range(3))
+        want = ev(adv if nlt == 1 else (adv + cons) / 2)
+        other = ev(cons if nlt == 1 else adv)
+        scale = np.abs(want[inner]).max()
+        err = np.abs(got - want)[inner].max() / scale
+        assert err < 1e-5, (name, err)
+        # the fields are not solenoidal, so the two forms differ
+        assert np.abs(got - other)[inner].max() / scale > 1e-2
+
+
+def test_integral_weights():
+    gx = np.array([0.0, 0.1, 0.3, 0.6, 1.0])
+    gy = np.array([-1.0, -0.5, 0.2, 0.4, 0.9, 1.0])
+    gz = np.array([0.0, 0.2, 0.5, 0.55, 0.9, 1.3, 2.0])
+    W = ns.integral_weights(gx, gy, gz)
+    f = np.random.default_rng(3).standard_normal(W.shape)
+    nx, ny, nz = W.shape
+    tot = 0.0
+    for k in range(1, nz - 1):           # interior (class_navier_3D.f90:2437-2446)
+        for j in range(1, ny - 1):
+            for i in range(1, nx - 1):
+                tot += f[i, j, k] * (gx[i + 1] - gx[i - 1]) * (gy[j + 1] - gy[j - 1]) * (gz[k + 1] - gz[k - 1]) / 8
+    for j in range(1, ny - 1):           # x faces
+        for k in range(1, nz - 1):
+            dyz = (gy[j + 1] - gy[j - 1]) * (gz[k + 1] - gz[k - 1]) / 4
+            tot += f[0, j, k] * dyz * (gx[1] - gx[0]) / 2 + f[-1, j, k] * dyz * (gx[-1] - gx[-2]) / 2
+    for i in range(1, nx - 1):           # y faces
+        for k in range(1, nz - 1):
+            dxz = (gx[i + 1] - gx[i - 1]) * (gz[k + 1] - gz[k - 1]) / 4
+            tot += f[i, 0, k] * dxz * (gy[1] - gy[0]) / 2 + f[i, -1, k] * dxz * (gy[-1] - gy[-2]) / 2
+    for i in range(1, nx - 1):           # z faces
+        for j in range(1, ny - 1):
+            dxy = (gx[i + 1] - gx[i - 1]) * (gy[j + 1] - gy[j - 1]) / 4
+            tot += f[i, j, 0] * dxy * (gz[1] - gz[0]) / 2 + f[i, j, -1] * dxy * (gz[-1] - gz[-2]) / 2
+    assert abs((W * f).sum() - tot) < 1e-13 * max(1.0, abs(tot))
+
+
+def _channel(n=(16, 24, 12), amp=0.1):
+    from mflops import synthetic
+    return synthetic.channel_problem(*n, amp=amp)
+
+
+@pytest.mark.parametrize("flow", [False, True])
+def test_laminar_channel_is_steady(flow):
+    gx, gy, gz, u, v, w, bc = _channel()
+    U = np.asfortranarray(np.broadcast_to(1.0 - gy[None, :, None] ** 2, u.shape).copy())
+    re, dt = 3000.0, 1e-3
+    s = ns.NSOracle(gx, gy, gz, re, dt)
+    s.set_state(U, 0 * U, 0 * U)
+    for c in range(3):
+        s.bc["uvw"[c]] = bc[c]
+    if flow:
+        s.flow = ns.FlowRate(g0=2.0 / re)
+    else:
+        s.force["u"] = np.asfortranarray(np.full(U.shape, 2.0 / re))
+    for _ in range(8):
+        s.step()
+    uu, vv, ww = s.velocity()
+    inner = (slice(1, -1),) * 3
+    assert np.abs(uu - U)[inner].max() < 1e-12
+    assert max(np.abs(vv)[inner].max(), np.abs(ww)[inner].max()) < 1e-12
+    if flow:
+        assert abs(s.flow.g - 2.0 / re) < 1e-9
+
+
+def _mms(n, sign, re=10.0, dt=1e-2, steps=30):
+    x, y, z = sp.symbols("x y z")
+    X = (x, y, z)
+    psi = sp.sin(sp.pi * x) ** 2 * sp.sin(sp.pi * y) ** 2 * sp.cos(sp.pi * z / 2) / sp.pi
+    U = [sp.diff(psi, y), -sp.diff(psi, x), sp.Integer(0)]
+    P = sp.cos(sp.pi * x) * sp.cos(sp.pi * y) * sp.sin(sp.pi * z)
+    F = []
+    for c in range(3):
+        adv = sum(U[a] * sp.diff(U[c], X[a]) for a in range(3))
+        lap = sum(sp.diff(U[c], X[a], 2) for a in range(3))
+        F.append(sign * adv + sp.diff(P, X[c]) - lap / re)
+    g = o.grid_uniform(n, 0.0, 1.0)
+    XX, YY, ZZ = np.meshgrid(g, g, g, indexing="ij")
+    ev = lambda e: np.asfortranarray(np.broadcast_to(
+        sp.lambdify(X, e, "numpy")(XX, YY, ZZ), XX.shape).astype(float))
+    u = [ev(e) for e in U]
+    s = ns.NSOracle(g, g, g, re, dt)
+    s.set_state(*u)
+    for c in range(3):
+        s.force["uvw"[c]] = ev(F[c])
+        q = u[c]
+        s.bc["uvw"[c]] = (np.asfortranarray(np.stack([q[0, 1:-1, 1:-1], q[-1, 1:-1, 1:-1]], -1)),
+                          np.asfortranarray(np.stack([q[1:-1, 0, 1:-1], q[1:-1, -1, 1:-1]], -1)),
+                          np.asfortranarray(np.stack([q[1:-1, 1:-1, 0], q[1:-1, 1:-1, -1]], -1)))
+    for _ in range(steps):
+        s.step()
+    vel = s.velocity()
+    inner = (slice(1, -1),) * 3
+    return max(np.abs(vel[c] - u[c])[inner].max() for c in range(3)) / max(np.abs(q).max() for q in u)
+
+
+@pytest.mark.slow
+def test_manufactured_steady_flow():
+    e16, e24 = _mms(16, 1), _mms(24, 1)
+    flipped = _mms(16, -1)
+    assert e16 < 0.02 and e24 < 0.6 * e16, (e16, e24)
+    assert flipped > 0.1, flipped
+
+
+def test_projection_reduces_divergence():
+    gx, gy, gz, u, v, w, bc = _channel()
+    s = ns.NSOracle(gx, gy, gz, 3000.0, 1e-3)
+    s.set_state(u, v, w)
+    for c in range(3):
+        s.bc["uvw"[c]] = bc[c]
+    s.flow = ns.FlowRate(g0=2.0 / 3000.0)
+    d0 = s.der(u, 0) + s.der(v, 1) + s.der(w, 2)
+    for _ in range(3):
+        s.step()
+    uu, vv, ww = s.velocity()
+    d = s.der(uu, 0) + s.der(vv, 1) + s.der(ww, 2)
+    inner = (slice(2, -2),) * 3
+    assert np.abs(d[inner]).max() < 0.2 * np.abs(d0[inner]).max()
+
+
+# --------------------------------------------------------------------- GPU --
+
+def _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt, **kw):
+    s = mf.NavierStokes(gx, gy, gz, re, dt, **kw)
+    for c in range(3):
+        s.set_bc(c, mf.BoundaryCondition(*[mf.to_device(b) for b in bc[c]]))
+    s.set_state(*[mf.to_device(f) for f in (u, v, w)])
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tou,top", [(2, 2), (3, 3), (1, 1)])
+def test_ns_steps_match_oracle(tou, top):
+    import mflops as mf
+    import torch
+    assert torch.cuda.is_available()
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((20, 24, 18))
+    re, dt, steps = 3000.0, 1e-3, 5
+    print("run ns tou=%d top=%d shape=%s steps=%d" % (tou, top, u.shape, steps), flush=True)
+    orc = ns.NSOracle(gx, gy, gz, re, dt, tou=tou, top=top)
+    orc.set_state(u, v, w)
+    for c in range(3):
+        orc.bc["uvw"[c]] = bc[c]
+    orc.flow = ns.FlowRate(g0=2.0 / re)
+    for _ in range(steps):
+        orc.step()
+    dev = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt, tou=tou, top=top)
+    dev.set_flow_rate(True, 2.0 / re)
+    dev.step(steps, mf.MF_FP64)
+    got = [mf.to_host(t) for t in dev.get()]
+    want = list(orc.velocity()) + [orc.pressure(), orc.phi_latest()]
+    for name, a, b in zip(("u", "v", "w", "p", "phi"), got, want):
+        scale = max(np.abs(b).max(), 1e-30)
+        err = np.abs(a - b).max() / scale
+        print("  %s: max|gpu-oracle|/max|oracle| = %.2e" % (name, err), flush=True)
+        assert err < 1e-9, (name, err)
+    t, g = dev.info()
+    assert abs(t - steps * dt) < 1e-12
+    assert abs(g - orc.flow.g) < 1e-9 * abs(orc.flow.g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [1, 2])
+def test_ns_mixed_velocity_solves(prec):
+    import mflops as mf
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((20, 24, 18))
+    re, dt, steps = 3000.0, 1e-3, 5
+    print("run ns mixed precision=%d" % prec, flush=True)
+    a = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt)
+    b = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt)
+    for s in (a, b):
+        s.set_flow_rate(True, 2.0 / re)
+    a.step(steps, mf.MF_FP64)
+    b.step(steps, prec, tol=1e-13)
+    for name, x, y in zip(("u", "v", "w", "p"), a.get()[:4], b.get()[:4]):
+        xa, ya = mf.to_host(x), mf.to_host(y)
+        err = np.abs(xa - ya).max() / max(np.abs(xa).max(), 1e-30)
+        print("  %s: mixed vs fp64 %.2e" % (name, err), flush=True)
+        assert err < 1e-9, (name, err)
