@@ -46,6 +46,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-solves", type=int, default=2)
     ap.add_argument("--workload", choices=("helmholtz", "influence"), default="helmholtz")
+    ap.add_argument("--ns-steps", type=int, default=10, help="timed NS steps (0: skip)")
+    ap.add_argument("--ns-warmup", type=int, default=3)
+    ap.add_argument("--ns-shape", type=int, nargs=3, default=(192, 256, 192))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -178,7 +181,57 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"], errs = cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx[best])
         out.update(errs)
+    if args.ns_steps > 0:
+        del sc, gd, u64, umx
+        torch.cuda.empty_cache()
+        out["ns_steps_per_s"] = ns_segment(args)
     print(json.dumps(out), flush=True)
+
+
+def ns_segment(args, cpu_steps=0):
+    """NS prediction-projection steps/s on the 192x256x192 channel (Re=3000,
+    dt=1e-3, tanh stretch 2.5 in y): args.ns_warmup untimed then
+    args.ns_steps timed steps, for FP64 velocity solves and for the split-bf16
+    mixed velocity solves (tol 1e-12); the pressure solve is FP64 in both."""
+    import numpy as np
+    import torch
+
+    import mflops as mf
+    from mflops import synthetic
+
+    nx, ny, nz = args.ns_shape
+    re, dt = 3000.0, 1e-3
+    log("ns: building %dx%dx%d channel" % (nx, ny, nz))
+    gx, gy, gz, u, v, w, bc = synthetic.channel_problem(nx, ny, nz, re=re)
+    res = {"unit": "steps/s", "config": {"workload": "channel-%dx%dx%d-tanh2.5" % (nx, ny, nz),
+                                         "re": re, "dt": dt, "tou": 2, "top": 2, "nlt": 2,
+                                         "pt": 1, "warmup": args.ns_warmup,
+                                         "steps": args.ns_steps,
+                                         "pressure_solve": "fp64"}}
+    fields = [mf.to_device(f) for f in (u, v, w)]
+    bcs = [mf.BoundaryCondition(*[mf.to_device(b) for b in bc[c]]) for c in range(3)]
+    finals = {}
+    for name, prec in (("fp64", mf.MF_FP64), ("mixed_bf16x3", mf.MF_MIXED_BF16X3)):
+        s = mf.NavierStokes(gx, gy, gz, re, dt)
+        for c in range(3):
+            s.set_bc(c, bcs[c])
+        s.set_state(*fields)
+        s.set_flow_rate(True, 2.0 / re)
+        s.step(args.ns_warmup, prec, 1e-12)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(args.ns_steps, prec, 1e-12)
+        torch.cuda.synchronize()
+        ts = (time.perf_counter() - t0) / args.ns_steps
+        res[name] = round(1.0 / ts, 3)
+        res[name + "_ms_per_step"] = round(ts * 1e3, 4)
+        finals[name] = mf.to_host(s.get()[0])
+        log("ns %s: %.3f ms/step" % (name, ts * 1e3))
+        del s
+    a, b = finals["fp64"], finals["mixed_bf16x3"]
+    res["u_mixed_vs_fp64"] = float(np.abs(a - b).max() / np.abs(a).max())
+    res["value"] = res["mixed_bf16x3"]
+    return res
 
 
 def cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx):

@@ -101,6 +101,12 @@ int mf_der3d(const mf_der *dc, int axis, int order, const double *x, double *out
              int nx, int ny, int nz, void *stream);
 int mf_der3d_host(const mf_der *dc, int axis, int order, const double *x, double *out,
                   int nx, int ny, int nz);
+/* mf_der3d of nfields (1..8) fields of one shape in one launch: x[f] -> out[f]
+ * (host arrays of device pointers; no output may alias an input).  The
+ * line recurrences are latency-bound, so fields batched together run in
+ * about the time of one. */
+int mf_der3d_batch(const mf_der *dc, int axis, int order, int nfields, const double *const *x,
+                   double *const *out, int nx, int ny, int nz, void *stream);
 
 /* ---------------------------------------------------------------------
  * Helmholtz / Poisson solver:  d2u/dx2 + d2u/dy2 + d2u/dz2 + sigma u = g.

@@ -42,43 +42,48 @@ __device__ __forceinline__ bool on_edge(long q, const Dims &d) {
   return nb >= 2;
 }
 
-// P_j = q u_j
-__global__ void __launch_bounds__(256) prod3_kernel(const double *__restrict__ q,
-                                                    const double *__restrict__ u,
+// Products q u_j of the velocity components: uu, uv, uw, vv, vw, ww
+__global__ void __launch_bounds__(256) prod6_kernel(const double *__restrict__ u,
                                                     const double *__restrict__ v,
                                                     const double *__restrict__ w,
-                                                    double *__restrict__ P0, double *__restrict__ P1,
-                                                    double *__restrict__ P2, long N) {
+                                                    double *__restrict__ uu, double *__restrict__ uv,
+                                                    double *__restrict__ uw, double *__restrict__ vv,
+                                                    double *__restrict__ vw, double *__restrict__ ww,
+                                                    long N) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
-  const double a = q[t];
-  P0[t] = a * u[t];
-  P1[t] = a * v[t];
-  P2[t] = a * w[t];
+  const double a = u[t], b = v[t], c = w[t];
+  uu[t] = a * a; uv[t] = a * b; uw[t] = a * c;
+  vv[t] = b * b; vw[t] = b * c; ww[t] = c * c;
 }
 
-// N = (u d0 + v d1 + w d2)/2 + e0/2 + e1/2 + e2/2 (nlt=2) or u d0 + v d1 + w d2 (nlt=1)
-__global__ void __launch_bounds__(256) nl_kernel(const double *__restrict__ u,
-                                                 const double *__restrict__ v,
-                                                 const double *__restrict__ w,
-                                                 const double *__restrict__ d0,
-                                                 const double *__restrict__ d1,
-                                                 const double *__restrict__ d2,
-                                                 const double *__restrict__ e0,
-                                                 const double *__restrict__ e1,
-                                                 const double *__restrict__ e2,
-                                                 double *__restrict__ out, long N, int nlt) {
+struct NlArgs {
+  const double *u, *v, *w;
+  const double *g[3][3];   // g[axis][comp] = d q_comp / d x_axis
+  const double *e[3][3];   // e[axis][comp] = d (q_comp u_axis) / d x_axis (nlt=2)
+  double *out[3];
+  long N;
+  int nlt;
+};
+
+// N_c = (u q_x + v q_y + w q_z)/2 + (q u)_x/2 + (q v)_y/2 + (q w)_z/2 (nlt=2),
+// or the advective part alone (nlt=1), for c = u, v, w
+__global__ void __launch_bounds__(256) nl_kernel(NlArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N) return;
-  const double adv = u[t] * d0[t] + v[t] * d1[t] + w[t] * d2[t];
-  if (nlt == 1) {
-    out[t] = adv;
-  } else {
-    double r = adv * 0.5;
-    r = r + e0[t] * 0.5;
-    r = r + e1[t] * 0.5;
-    r = r + e2[t] * 0.5;
-    out[t] = r;
+  if (t >= a.N) return;
+  const double u = a.u[t], v = a.v[t], w = a.w[t];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double adv = u * a.g[0][c][t] + v * a.g[1][c][t] + w * a.g[2][c][t];
+    if (a.nlt == 1) {
+      a.out[c][t] = adv;
+    } else {
+      double r = adv * 0.5;
+      r = r + a.e[0][c][t] * 0.5;
+      r = r + a.e[1][c][t] * 0.5;
+      r = r + a.e[2][c][t] * 0.5;
+      a.out[c][t] = r;
+    }
   }
 }
 
@@ -327,7 +332,7 @@ struct mf_ns {
   mf_solver3d *scu = nullptr, *scp = nullptr;
   void *mem = nullptr;
   double *q[3][NT], *p[NT], *phi[NT], *nl[3][NT], *dphi[NT][3];
-  bool nl_ok[3][NT], dphi_ok[NT];
+  bool nl_ok[NT], dphi_ok[NT];
   double *bc0[3], *bcw[3], *bcphi;
   double *force[3];
   bool has_force[3] = {false, false, false};
@@ -336,7 +341,7 @@ struct mf_ns {
   double *W = nullptr, *fstate = nullptr, *part = nullptr;
   double vol = 0.0;
   int nparts = 0;
-  double *T[9];
+  double *T[24];
   int it[NT] = {0, 1, 2, 3};
   double time = 0.0;
   mf::ns::Dims d{0, 0, 0};
@@ -369,21 +374,42 @@ int der(mf_ns *s, int axis, const double *x, double *out, hipStream_t st) {
   return mf_der3d(s->dc[axis], axis, 1, x, out, s->nx, s->ny, s->nz, st);
 }
 
-// nonlinear term of component c at level L (cached)
-int nonlinear(mf_ns *s, int c, int L, hipStream_t st) {
-  if (s->nl_ok[c][L]) return MF_OK;
+int der3(mf_ns *s, int axis, const double *x0, const double *x1, const double *x2, double *o0,
+         double *o1, double *o2, hipStream_t st) {
+  const double *x[3] = {x0, x1, x2};
+  double *o[3] = {o0, o1, o2};
+  return mf_der3d_batch(s->dc[axis], axis, 1, 3, x, o, s->nx, s->ny, s->nz, st);
+}
+
+// nonlinear terms of the three components at level L (cached): the 9
+// velocity gradients and the 9 product derivatives in 6 batched launches
+int nonlinear(mf_ns *s, int L, hipStream_t st) {
+  if (s->nl_ok[L]) return MF_OK;
   const long N = s->N;
-  double *q = s->q[c][L], *u = s->q[0][L], *v = s->q[1][L], *w = s->q[2][L];
-  double **T = s->T;
-  for (int a = 0; a < 3; ++a) RC(der(s, a, q, T[a], st));
+  double *u = s->q[0][L], *v = s->q[1][L], *w = s->q[2][L];
+  double **T = s->T;  // T[3a + c] gradients, T[9..14] products, T[15 + 3a + c] product derivatives
+  for (int a = 0; a < 3; ++a) RC(der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
+  NlArgs n;
+  std::memset(&n, 0, sizeof(n));
+  n.u = u; n.v = v; n.w = w;
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) n.g[a][c] = T[3 * a + c];
   if (s->nlt == 2) {
-    hipLaunchKernelGGL(prod3_kernel, dim3(nblk(N)), dim3(256), 0, st, q, u, v, w, T[3], T[4], T[5], N);
-    for (int a = 0; a < 3; ++a) RC(der(s, a, T[3 + a], T[6 + a], st));
+    double *uu = T[9], *uv = T[10], *uw = T[11], *vv = T[12], *vw = T[13], *ww = T[14];
+    hipLaunchKernelGGL(prod6_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)u,
+                       (const double *)v, (const double *)w, uu, uv, uw, vv, vw, ww, N);
+    RC(der3(s, 0, uu, uv, uw, T[15], T[16], T[17], st));  // (q u)_x, q = u, v, w
+    RC(der3(s, 1, uv, vv, vw, T[18], T[19], T[20], st));  // (q v)_y
+    RC(der3(s, 2, uw, vw, ww, T[21], T[22], T[23], st));  // (q w)_z
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) n.e[a][c] = T[15 + 3 * a + c];
   }
-  hipLaunchKernelGGL(nl_kernel, dim3(nblk(N)), dim3(256), 0, st, u, v, w, T[0], T[1], T[2], T[6],
-                     T[7], T[8], s->nl[c][L], N, s->nlt);
+  for (int c = 0; c < 3; ++c) n.out[c] = s->nl[c][L];
+  n.N = N;
+  n.nlt = s->nlt;
+  hipLaunchKernelGGL(nl_kernel, dim3(nblk(N)), dim3(256), 0, st, n);
   NSCHK(hipGetLastError());
-  s->nl_ok[c][L] = true;
+  s->nl_ok[L] = true;
   return MF_OK;
 }
 
@@ -398,9 +424,8 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
   extrap_coefs(s->top - 1, cp);
   const int lv[3] = {L0, L1, L2};
   // nonlinear terms of the levels the extrapolation reads
-  for (int c = 0; c < 3; ++c)
-    for (int m = 0; m < 3; ++m)
-      if (cv[m] != 0.0) RC(nonlinear(s, c, lv[m], st));
+  for (int m = 0; m < 3; ++m)
+    if (cv[m] != 0.0) RC(nonlinear(s, lv[m], st));
   // constant-flow-rate force from extrapol_v(u)
   if (s->flow) {
     hipLaunchKernelGGL(wsum_kernel, dim3(s->nparts), dim3(256), 0, st, (const double *)s->W,
@@ -442,7 +467,6 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
     const double *bc = s->bcw[c];
     RC(mf_solver3d_solve(s->scu, s->T[0], -s->re * fac1, s->q[c][n1], bc, bc + sx, bc + sx + sy,
                          precision, tol, 0, nullptr, st));
-    s->nl_ok[c][n1] = false;
   }
   // pressure: rhs_p = fac1 u*, fphi = div rhs_p, Neumann values, solve
   double **T = s->T;
@@ -470,7 +494,7 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
                      (const double *)s->dphi[n1][1], (const double *)s->dphi[n1][2], fac1, s->re,
                      s->p[n1], s->q[0][n1], s->q[1][n1], s->q[2][n1], N, s->d);
   NSCHK(hipGetLastError());
-  for (int c = 0; c < 3; ++c) s->nl_ok[c][n1] = false;
+  s->nl_ok[n1] = false;
   // rotate the time levels
   const int first = s->it[0];
   for (int m = 0; m < NT - 1; ++m) s->it[m] = s->it[m + 1];
@@ -523,7 +547,7 @@ extern "C" int mf_ns_create(const double *gx, int nx, const double *gy, int ny, 
   // one allocation: fields, then bc arrays, weights, scalars
   const long N = s->N;
   const long Na = (N + 31) / 32 * 32, Ba = (s->nbc + 31) / 32 * 32;
-  const int nfields = 3 * NT + NT + NT + 3 * NT + 3 * NT + 3 + 9 + 1;
+  const int nfields = 3 * NT + NT + NT + 3 * NT + 3 * NT + 3 + 24 + 1;
   s->nparts = 1024;
   const size_t total = (size_t)nfields * Na + 7 * Ba + 64 + s->nparts;
   hipError_t e = hipMalloc(&s->mem, total * sizeof(double));
@@ -539,7 +563,7 @@ extern "C" int mf_ns_create(const double *gx, int nx, const double *gy, int ny, 
   for (int c = 0; c < 3; ++c) for (int L = 0; L < NT; ++L) s->nl[c][L] = take(Na);
   for (int L = 0; L < NT; ++L) for (int a = 0; a < 3; ++a) s->dphi[L][a] = take(Na);
   for (int c = 0; c < 3; ++c) s->force[c] = take(Na);
-  for (int k = 0; k < 9; ++k) s->T[k] = take(Na);
+  for (int k = 0; k < 24; ++k) s->T[k] = take(Na);
   s->W = take(Na);
   for (int c = 0; c < 3; ++c) s->bc0[c] = take(Ba);
   for (int c = 0; c < 3; ++c) s->bcw[c] = take(Ba);
@@ -576,9 +600,7 @@ extern "C" int mf_ns_create(const double *gx, int nx, const double *gy, int ny, 
         if (nb < 2) vol += w1(gx, nx, i) * w1(gy, ny, j) * w1(gz, nz, k);
       }
   s->vol = vol;
-  for (int c = 0; c < 3; ++c)
-    for (int L = 0; L < NT; ++L) s->nl_ok[c][L] = false;
-  for (int L = 0; L < NT; ++L) s->dphi_ok[L] = false;
+  for (int L = 0; L < NT; ++L) s->nl_ok[L] = s->dphi_ok[L] = false;
   *out = s;
   return MF_OK;
 }
@@ -623,7 +645,7 @@ extern "C" int mf_ns_set_state(mf_ns *s, const double *u, const double *v, const
     for (int L = 0; L < NT; ++L) {
       NSCHK(hipMemcpyAsync(s->q[c][L], src[c], s->N * sizeof(double), hipMemcpyDeviceToDevice, st));
       hipLaunchKernelGGL(zero_edges_kernel, dim3(nblk(ne)), dim3(256), 0, st, s->q[c][L], s->d);
-      s->nl_ok[c][L] = false;
+      s->nl_ok[L] = false;
     }
   for (int L = 0; L < NT; ++L) {
     NSCHK(hipMemsetAsync(s->p[L], 0, s->N * sizeof(double), st));

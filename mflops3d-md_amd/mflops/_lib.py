@@ -30,6 +30,8 @@ SIGNATURES = {
     "mf_der_lines": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_longlong, _vp]),
     "mf_der3d": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mf_der3d_host": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "mf_der3d_batch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int,
+                                 C.c_int, _vp]),
     "mf_solver3d_create": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int,
                                      C.POINTER(_vp)]),
     "mf_solver3d_destroy": (C.c_int, [_vp]),
