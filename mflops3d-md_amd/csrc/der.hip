@@ -292,25 +292,35 @@ __global__ void __launch_bounds__(64 * SEG_KMAX) der3d_seg_kernel(FieldSet fs, i
   int blk = blockIdx.x;
   SegLines L;
   L.sa = sa; L.sb = sb; L.s = s;
+  // one call site of seg_solve: pick the scheme, the line family and the
+  // field offset of this block, then solve
+  SegTab T = full;
+  long off = 0;
+  bool solve = true;
   if (blk < b_full) {
     if (mode) { L.a0 = 1; L.na = na - 2; L.b0 = 1; L.nb = nb - 2; }
     else { L.a0 = 0; L.na = na; L.b0 = 0; L.nb = nb; }
-    L.nlines = L.na * L.nb;
-    seg_solve(x, out, full, L, blk, lds);
-    return;
-  }
-  blk -= b_full;
-  const int fam_blocks[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-#pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    if (blk < fam_blocks[f]) {
-      if (f < 2) { L.a0 = 1; L.na = na - 2; L.b0 = f ? nb - 1 : 0; L.nb = 1; }
-      else { L.a0 = (f == 3) ? na - 1 : 0; L.na = 1; L.b0 = 1; L.nb = nb - 2; }
-      L.nlines = L.na * L.nb;
-      seg_solve(x + s, out + s, sub, L, blk, lds);
-      return;
+  } else {
+    blk -= b_full;
+    if (blk < nb4.x + nb4.y) {
+      const int f = blk >= nb4.x;
+      blk -= f ? nb4.x : 0;
+      L.a0 = 1; L.na = na - 2; L.b0 = f ? nb - 1 : 0; L.nb = 1;
+    } else if (blk < nb4.x + nb4.y + nb4.z + nb4.w) {
+      blk -= nb4.x + nb4.y;
+      const int f = blk >= nb4.z;
+      blk -= f ? nb4.z : 0;
+      L.a0 = f ? na - 1 : 0; L.na = 1; L.b0 = 1; L.nb = nb - 2;
+    } else {
+      solve = false;
     }
-    blk -= fam_blocks[f];
+    T = sub;
+    off = s;  // the sub-line scheme acts on points 1..n-2
+  }
+  if (solve) {
+    L.nlines = L.na * L.nb;
+    seg_solve(x + off, out + off, T, L, blk, lds);
+    return;
   }
   // zeros: the four two-boundary-plane lines and both ends of the sub lines
   const int nt = blockDim.x;
@@ -367,11 +377,12 @@ bool mf::der_host_tables(const mf_der *dc, int order, int sub, const double **dl
 }
 
 namespace {
-// pack one scheme: dr row-major (7n) | p | f | d | e | c | 1/c | A | B | C | D
-// (A..D: the segment homogeneous responses of seg_tables)
+// pack one scheme: dr row-major (7n) | p | f | d | e | c | 1/c, then the
+// segmented solve's tables (seg_tables): rows (SEG_ROW n) | ab (2n) | cd (2n)
+constexpr int kPackPerPoint = 13 + mf::SEG_ROW + 4;
 void pack_scheme(int n, const std::vector<double> &dl, const std::vector<double> &dr,
                  std::vector<double> &dst) {
-  dst.assign((size_t)mf::SEG_TAB * n, 0.0);
+  dst.assign((size_t)kPackPerPoint * n, 0.0);
   for (int i = 0; i < n; ++i)
     for (int m = 0; m < 7; ++m) dst[(size_t)7 * i + m] = dr[i + (size_t)n * m];
   double *p = &dst[(size_t)7 * n];
@@ -384,8 +395,10 @@ void pack_scheme(int n, const std::vector<double> &dl, const std::vector<double>
     p[5 * n + i] = 1.0 / dl[2 * (size_t)n + i];
   }
   // alpha = dl(1,1) and beta = dl(n,5) of the factored matrix
-  mf::seg_tables(n, p, p + n, p + 2 * n, p + 3 * n, p + 5 * n, dl[0], dl[4 * (size_t)n + n - 1],
-                 p + 6 * n, p + 7 * n, p + 8 * n, p + 9 * n);
+  double *rows = p + 6 * n;
+  mf::seg_tables(n, dr.data(), p, p + n, p + 2 * n, p + 3 * n, p + 5 * n, dl[0],
+                 dl[4 * (size_t)n + n - 1], rows, rows + mf::SEG_ROW * n,
+                 rows + (mf::SEG_ROW + 2) * n);
 }
 #define HIPCHK(call)                                                             \
   do {                                                                           \
@@ -528,10 +541,16 @@ extern "C" int mf_der3d_batch(const mf_der *dc, int axis, int order, int nfields
     blocks = b_full + b_sub + 1;
   }
   hipStream_t st = (hipStream_t)stream;
-  if (mf::seg_count(F.n) <= mf::SEG_KMAX && mf::seg_lds_doubles(F.n) * sizeof(double) <= 65536) {
-    // segmented solves: 64 lines per block, one wavefront per segment
-    const mf::SegTab SF{F.dr, F.alpha, F.beta, F.n};
-    const mf::SegTab SS{S.dr, S.alpha, S.beta, S.n};
+  if (mf::seg_count(F.n) <= mf::SEG_KMAX) {
+    // segmented solves: 64 lines per block, one wavefront per segment; the
+    // row tables follow the packed 1/c
+    auto seg_of = [](const mf::LineTab &t) {
+      if (!t.rc) return mf::SegTab{nullptr, nullptr, nullptr, 0.0, 0.0, 0};
+      const double *rows = t.rc + t.n;
+      return mf::SegTab{rows, rows + mf::SEG_ROW * t.n, rows + (mf::SEG_ROW + 2) * t.n,
+                        t.alpha, t.beta, t.n};
+    };
+    const mf::SegTab SF = seg_of(F), SS = seg_of(S);
     int4 nb4 = make_int4(0, 0, 0, 0);
     int sb_full, sblocks;
     if (!mode) {

@@ -9,11 +9,10 @@
 // the FP32 MFMA path on gfx950 (bf16 16x16x32 takes 16 cycles, f32 16x16x4
 // takes 32 for 1/8 of the work).
 //
-// The eigenvector operand is split once on the host and stored as
-// [part][row][k] (k contiguous, padded to a multiple of 32); the field
-// operand is split while it is staged to LDS.  Tiles, wave layout, operand
-// swap (lanes contiguous in m for the column-major stores) and epilogues are
-// those of gemm_mode_kernel.
+// The eigenvector operand is split once on the host (k padded to a multiple
+// of 64); the field operand is split while it is staged to LDS.  Tiles, wave
+// layout, operand swap (lanes contiguous in m for the column-major stores)
+// and epilogues are those of gemm_mode_kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -34,133 +33,150 @@ __device__ __forceinline__ void split3(float x, __bf16 &p0, __bf16 &p1, __bf16 &
 // WLEFT: A is the eigenvector matrix (x-transform), B the field; otherwise
 // A is the field and B the eigenvector matrix (y/z transforms).
 // Wsplit: bf16 [3][KP/8][WROWS][8] (part, 8-column group of k, row, k within
-// the group), zero beyond the matrix, with WROWS a multiple of the tile side
-// along the eigenvector dimension, so its tile loads need no guards; lanes
-// read consecutive rows (coalesced) and write them to LDS rows 80 bytes apart
-// (bank-conflict free).  Loads are branch-free (clamped addresses, k >= K selected to 0)
-// and the next k-step's tile is fetched into registers while the current one
-// is multiplied.
+// the group), zero beyond the matrix, KP a multiple of 64 and WROWS a
+// multiple of the tile side along the eigenvector dimension, so its loads
+// need no guards.  A wave's eigenvector fragments (its 64 rows, three parts)
+// are loaded straight from global memory (L2) into registers in MFMA operand
+// layout, one k-step ahead; only the field operand goes through LDS, split
+// into its three parts while staged, in two buffers so that one barrier per
+// k-step suffices.  Every load is unconditional (field addresses clamped
+// into the array, 32-bit offsets from the kernel-argument base), so the
+// compiler waits for exactly the loads each instruction needs.
 // The block has BM*BN/4096 wavefronts, each computing a 64 x 64 tile.
 template <typename TF, typename TC, int BM, int BN, bool WLEFT, int EPI>
-__global__ void __launch_bounds__(BM * BN / 64)
+__global__ void __launch_bounds__(256, 2)
     gemm_bf3_kernel(const TF *__restrict__ F, long ldf, long sF, const __bf16 *__restrict__ Wsplit,
                     int KP, int WROWS, TC *__restrict__ C, long ldc, long sC, int M, int N, int K,
                     EpiArgs ep) {
-  constexpr int NT = BM * BN / 64;    // threads
+  constexpr int NT = 256;
   constexpr int BK = 32;
-  constexpr int PAD = 8;  // bf16 elements of row padding (16 B)
-  constexpr int RW = WLEFT ? BM : BN;  // eigenvector rows per tile
-  constexpr int RF = WLEFT ? BN : BM;  // field rows (m or n) per tile
-  constexpr int NW = 3 * RW * (BK / 8) / NT;  // 16-byte eigenvector chunks per thread
-  constexpr int NG = RF * (BK / 8) / NT;      // groups of 8 field values per thread
-  static_assert(NW * NT == 3 * RW * (BK / 8) && NG * NT == RF * (BK / 8), "tile");
-  __shared__ __attribute__((aligned(16))) __bf16 Ws[3][RW][BK + PAD];
-  __shared__ __attribute__((aligned(16))) __bf16 Fs[3][RF][BK + PAD];
+  constexpr int PAD = 8;                  // bf16 elements of row padding (16 B)
+  constexpr int RF = WLEFT ? BN : BM;     // field rows per tile
+  constexpr int NG = RF * (BK / 8) / NT;  // groups of 8 field values per thread
+  static_assert(BM * BN == NT * 64 && NG * NT == RF * (BK / 8), "tile");
+  __shared__ __attribute__((aligned(16))) __bf16 Fs[2][3][RF][BK + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long m0 = (long)blockIdx.x * BM, n0 = (long)blockIdx.y * BN;
   F += (long)blockIdx.z * sF;
   if (EPI == EPI_STORE || EPI == EPI_DIVIDE) C += (long)blockIdx.z * sC;
   constexpr int WM = BM / 64;
   const int wm = (wave % WM) * 64, wn = (wave / WM) * 64;
-  const long w0 = WLEFT ? m0 : n0;   // first eigenvector row of the tile
-  const long f0 = WLEFT ? n0 : m0;   // first field row of the tile
-  const long FR = WLEFT ? N : M;     // field rows
+  const int wr0 = (int)(WLEFT ? m0 + wm : n0 + wn);  // first eigenvector row of this wave
+  const long f0 = WLEFT ? n0 : m0;                   // first field row of the tile
+  const long FR = WLEFT ? N : M;                     // field rows
   f4_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0, 0, 0, 0};
 
-  bf16x8_t wreg[NW];
-  TF freg[NG][8];
+  // eigenvector fragments (byte offsets): part p, 16-row tile q of k-step t:
+  // lane l holds row wr0 + 16 q + l%16, k = 32 t + 8 (l/16) .. +7
+  const char *Wb = reinterpret_cast<const char *>(Wsplit);
+  const unsigned wlane = ((unsigned)(lane >> 4) * WROWS + wr0 + (lane & 15)) * 16u;
+  const unsigned wpart = (unsigned)(KP / 8) * WROWS * 16u, wstep = (unsigned)WROWS * 64u;
+  auto loadW = [&](int t, bf16x8_t (&wf)[3][4]) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        wf[p][q] = *reinterpret_cast<const bf16x8_t *>(
+            Wb + (wlane + p * wpart + (unsigned)t * wstep + q * 256u));
+  };
   // field element (row r, column k) of the tile: WLEFT -> F[k + ldf r]
-  // (k contiguous, 4 threads per row); else F[r + ldf k] (lanes along r)
+  // (k contiguous, 4 threads per row); else F[r + ldf k] (lanes along r).
+  // Columns k >= K are read at K-1 and replaced by zero when split.
   auto frow = [&](int g) { return WLEFT ? (tid + NT * g) / (BK / 8) : (tid + NT * g) % RF; };
   auto fkq = [&](int g) { return WLEFT ? ((tid + NT * g) % (BK / 8)) * 8 : ((tid + NT * g) / RF) * 8; };
-  auto load = [&](int kt) {
+  const char *Fb = reinterpret_cast<const char *>(F);
+  const unsigned ldfu = (unsigned)ldf;
+  unsigned frb[NG];
 #pragma unroll
-    for (int i = 0; i < NW; ++i) {
-      const int e = tid + NT * i;
-      const int r = e % RW, c = (e / RW) % (BK / 8), p = e / ((BK / 8) * RW);
-      wreg[i] = *reinterpret_cast<const bf16x8_t *>(
-          Wsplit + (((size_t)p * (KP / 8) + kt / 8 + c) * WROWS + w0 + r) * 8);
-    }
+  for (int g = 0; g < NG; ++g) {
+    const long r0 = f0 + frow(g);
+    const unsigned r = (unsigned)(r0 < FR ? r0 : FR - 1);
+    frb[g] = WLEFT ? ldfu * r : r;
+  }
+  TF freg[NG][8];
+  auto loadF = [&](int t) {
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-      const long r0 = f0 + frow(g);
-      const long r = r0 < FR ? r0 : FR - 1;
+    for (int g = 0; g < NG; ++g)
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int k0 = kt + fkq(g) + q;
-        const long k = k0 < K ? k0 : K - 1;
-        freg[g][q] = WLEFT ? F[k + ldf * r] : F[r + ldf * k];
+        const int k0 = t * BK + fkq(g) + q;
+        const unsigned k = (unsigned)(k0 < K ? k0 : K - 1);
+        const unsigned e = WLEFT ? frb[g] + k : frb[g] + ldfu * k;
+        freg[g][q] = *reinterpret_cast<const TF *>(Fb + e * (unsigned)sizeof(TF));
       }
-    }
   };
-  load(0);
-  for (int kt = 0; kt < K; kt += BK) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NW; ++i) {
-      const int e = tid + NT * i;
-      const int r = e % RW, c = (e / RW) % (BK / 8), p = e / ((BK / 8) * RW);
-      *reinterpret_cast<bf16x8_t *>(&Ws[p][r][8 * c]) = wreg[i];
-    }
+  auto storeF = [&](int t, int buf) {
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       bf16x8_t v0, v1, v2;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const float x = (kt + fkq(g) + q < K) ? (float)freg[g][q] : 0.0f;
+        const float x = (t * BK + fkq(g) + q < K) ? (float)freg[g][q] : 0.0f;
         __bf16 a, b, c;
         split3(x, a, b, c);
         v0[q] = a; v1[q] = b; v2[q] = c;
       }
       const int r = frow(g), kq = fkq(g);
-      *reinterpret_cast<bf16x8_t *>(&Fs[0][r][kq]) = v0;
-      *reinterpret_cast<bf16x8_t *>(&Fs[1][r][kq]) = v1;
-      *reinterpret_cast<bf16x8_t *>(&Fs[2][r][kq]) = v2;
+      *reinterpret_cast<bf16x8_t *>(&Fs[buf][0][r][kq]) = v0;
+      *reinterpret_cast<bf16x8_t *>(&Fs[buf][1][r][kq]) = v1;
+      *reinterpret_cast<bf16x8_t *>(&Fs[buf][2][r][kq]) = v2;
     }
-    __syncthreads();
-    if (kt + BK < K) load(kt + BK);
-    // ---- six products per 16x16 tile; lane l holds k = 8(l/16)..+7
-    // D columns (m) come from the A tile, D rows (n) from the B tile
-    const int ko = 8 * (lane >> 4);
-    auto afrag = [&](int p, int row) {
-      if constexpr (WLEFT) return *reinterpret_cast<const bf16x8_t *>(&Ws[p][row][ko]);
-      else return *reinterpret_cast<const bf16x8_t *>(&Fs[p][row][ko]);
-    };
-    auto bfrag = [&](int p, int row) {
-      if constexpr (WLEFT) return *reinterpret_cast<const bf16x8_t *>(&Fs[p][row][ko]);
-      else return *reinterpret_cast<const bf16x8_t *>(&Ws[p][row][ko]);
-    };
-    bf16x8_t fa[3][4];
+  };
+  // ---- six products per 16x16 tile; lane l holds k = 8(l/16)..+7.
+  // D columns (m) come from the A operand, D rows (n) from the B operand.
+  const int ko = 8 * (lane >> 4);
+  const int fro = WLEFT ? wn : wm;  // this wave's first field row in the tile
+  auto step = [&](int buf, const bf16x8_t (&wf)[3][4]) {
+    bf16x8_t ff[3][4];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) fa[p][i] = afrag(p, wm + 16 * i + (lane & 15));
+      for (int i = 0; i < 4; ++i)
+        ff[p][i] = *reinterpret_cast<const bf16x8_t *>(&Fs[buf][p][fro + 16 * i + (lane & 15)][ko]);
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-      bf16x8_t fb[3];
-#pragma unroll
-      for (int p = 0; p < 3; ++p) fb[p] = bfrag(p, wn + 16 * ni + (lane & 15));
+    for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi) {
+        // parts sit 4 apart in both fragment arrays
+        const bf16x8_t *fa = WLEFT ? &wf[0][mi] : &ff[0][mi];
+        const bf16x8_t *fb = WLEFT ? &ff[0][ni] : &wf[0][ni];
         f4_t c = acc[mi][ni];
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[2], fa[0][mi], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1], fa[1][mi], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0], fa[2][mi], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1], fa[0][mi], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0], fa[1][mi], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0], fa[0][mi], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[8], fa[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[4], fa[4], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0], fa[8], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[4], fa[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0], fa[4], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0], fa[0], c, 0, 0, 0);
         acc[mi][ni] = c;
       }
-    }
+  };
+  // An even number of k-steps (KP is a multiple of 64; the steps past K
+  // multiply zeros); the prefetch after the last step re-reads it.
+  const int nk2 = ((K + 2 * BK - 1) / (2 * BK)) * 2;
+  bf16x8_t wA[3][4], wB[3][4];
+  loadF(0);
+  loadW(0, wA);
+  for (int t = 0; t < nk2; t += 2) {
+    storeF(t, 0);
+    loadF(t + 1);
+    __syncthreads();
+    loadW(t + 1, wB);
+    step(0, wA);
+    storeF(t + 1, 1);
+    loadF(min(t + 2, nk2 - 1));
+    __syncthreads();
+    loadW(min(t + 2, nk2 - 1), wA);
+    step(1, wB);
   }
   // ---- epilogue (as gemm_mode_kernel; f32 output layout).  The divide uses
-  // an FP32 reciprocal of the FP64 eigenvalue sum: the result is an FP32
-  // value of an approximate inner solve.  Index splits advance from one
-  // 32-bit division per lane.
+  // the FP32 reciprocal of an FP32 eigenvalue sum (lx+ly+sigma rounded once
+  // per row, plus lz): the result is an FP32 value of an approximate inner
+  // solve.  The null-mode test stays on the FP64 values.  Index splits
+  // advance from one 32-bit division per lane.
   int i1 = 0, i2 = 0;
   if (EPI == EPI_DIVIDE) {
     const int mb = (int)(m0 + wm + (lane & 15));
@@ -172,6 +188,20 @@ __global__ void __launch_bounds__(BM * BN / 64)
     const int nb = (int)(n0 + wn + 4 * (lane >> 4));
     jb = nb % ep.n2;
     kb = nb / ep.n2;
+  }
+  // this lane's 16 columns of lz (FP32) and their null-mode flags
+  float lzf[4][4];
+  bool nullz[4][4];
+  if (EPI == EPI_DIVIDE) {
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long n = n0 + wn + 4 * (lane >> 4) + ni * 16 + r;
+        const double lz = ep.lz[n < N ? n : N - 1];
+        lzf[ni][r] = (float)lz;
+        nullz[ni][r] = fabs(lz) < 1e-11;
+      }
   }
   auto uidx = [&](long m, int dn) {
     int j = jb + dn, k = kb;
@@ -186,10 +216,11 @@ __global__ void __launch_bounds__(BM * BN / 64)
       while (i1 >= ep.n1) { i1 -= ep.n1; ++i2; }
     }
     if (m >= M) continue;
-    double lxy = 0.0, big_xy = 0.0;
+    float lxy = 0.0f;
+    bool null_xy = false;
     if (EPI == EPI_DIVIDE) {
-      lxy = ep.lx[i1] + ep.ly[i2] + ep.sigma;
-      big_xy = fmax(fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2])), fabs(ep.sigma));
+      lxy = (float)(ep.lx[i1] + ep.ly[i2] + ep.sigma);
+      null_xy = fmax(fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2])), fabs(ep.sigma)) < 1e-11;
     }
     if (EPI == EPI_SCATTER_ADD) {
       // all loads of this row first, then the stores
@@ -223,10 +254,8 @@ __global__ void __launch_bounds__(BM * BN / 64)
         if (EPI == EPI_STORE) {
           C[m + ldc * n] = (TC)v;
         } else if (EPI == EPI_DIVIDE) {
-          const double lzn = ep.lz[n];
-          const double big = fmax(big_xy, fabs(lzn));
-          const float inv = __builtin_amdgcn_rcpf((float)(lxy + lzn));
-          C[m + ldc * n] = (TC)((big < 1e-11) ? 0.0f : v * inv);
+          const float inv = __builtin_amdgcn_rcpf(lxy + lzf[ni][r]);
+          C[m + ldc * n] = (TC)((null_xy && nullz[ni][r]) ? 0.0f : v * inv);
         } else {
           ep.u[uidx(m, dn)] = (double)v;
         }

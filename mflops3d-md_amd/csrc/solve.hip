@@ -463,7 +463,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->bandc = (int *)take((bandc.size() + 1) / 2);
     h->bctd = (int *)take(4);
     for (int a = 0; a < 3; ++a) {
-      h->kp[a] = (n[a] + 31) / 32 * 32;
+      h->kp[a] = (n[a] + 63) / 64 * 64;  // an even number of 32-wide k-steps
       const int t = gemm_tile(n[a]);
       h->wrows[a] = (n[a] + t - 1) / t * t;
       const size_t cnt = (3 * (size_t)h->wrows[a] * h->kp[a] + 3) / 4;  // bf16 -> doubles

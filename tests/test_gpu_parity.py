@@ -83,7 +83,12 @@ def test_derivatives_c1():
     _der_case(mf, (gx, gy, gz), u, 6, label="c1")
 
 
-@pytest.mark.parametrize("shape", [(9, 10, 11), (33, 17, 40), (70, 65, 10)])
+# line lengths cover one to four 32-row segments (9 .. 130, a 38-row last
+# segment at 70, sub-lines with fewer segments than the full lines at 39) and
+# a 340-point line, longer than the segmented solve's LDS tables allow, which
+# takes the one-lane-per-line kernel
+@pytest.mark.parametrize("shape", [(9, 10, 11), (33, 17, 40), (70, 65, 10), (130, 12, 9),
+                                   (39, 340, 38)])
 @pytest.mark.parametrize("so", [4, 6, 8])
 def test_derivatives_shapes(shape, so):
     mf, torch = _mf()
