@@ -32,6 +32,24 @@ sys.path.insert(0, os.path.join(ROOT, "mflops3d-md_amd"))
 METRIC = "Helmholtz solves/s @256³ mixed vs FP64 at ≤1e-12 rel residual; NS steps/s"
 PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0,
         "hbm_gbs": 8000.0}
+COPY_PROBE_BYTES = 256 << 20
+# HBM bytes per launch of the transform kernels, from the FETCH_SIZE / WRITE_SIZE
+# PMC runs (scaled by the copy probe); written by the profiling step
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "helmholtz256_pmc_traffic.md")
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of the kernels whose name starts with
+    kernel_prefix, from the JSON block of TRAFFIC_FILE; None if absent."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    start, end = text.find("```json"), text.rfind("```")
+    if start < 0 or end <= start:
+        return None
+    data = json.loads(text[start + 7:end])
+    return data.get("bytes_per_launch", {}).get(kernel_prefix)
 
 
 def log(msg):
@@ -51,6 +69,9 @@ def main():
     ap.add_argument("--ns-steps", type=int, default=10, help="timed NS steps (0: skip)")
     ap.add_argument("--ns-warmup", type=int, default=3)
     ap.add_argument("--ns-shape", type=int, nargs=3, default=(192, 256, 192))
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy a 256 MiB buffer 3 times first (the known-size kernel that "
+                         "scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -67,6 +88,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x %d bytes read and written" % COPY_PROBE_BYTES)
+        del src, dst
     n = args.n
     re, dt = 3000.0, 1e-3
     sigma = -re * 1.5 / dt
@@ -129,21 +158,25 @@ def main():
     scale = float(np.abs(a64[inner]).max())
     err_mx_64 = {name: float(np.abs(a[inner] - a64[inner]).max() / scale) for name, a in amx.items()}
 
-    def roof(p, key, peak, ops=1.0):
-        """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3)."""
+    def roof(p, key, peak, ops=1.0, traf=None):
+        """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3); traf:
+        measured HBM bytes per launch (PMC), or None."""
         ms, cnt, work = p[key]
         if cnt == 0:
             return None
         avg = ms / cnt * 1e-3
         ach = ops * work / cnt / avg / 1e12
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 4), "traffic": None,
+                "frac": round(ach / peak, 4), "traffic": traf,
                 "launches": cnt, "avg_launch_ms": round(ms / cnt, 4),
                 "flops_per_launch": ops * work / cnt}
 
-    roofs = {"fp32": roof(res["fp32"][2], "transform_fp32", PEAK["fp32_mfma_tflops"]),
-             "bf16x3": roof(res["bf16x3"][2], "transform_fp32", PEAK["bf16_mfma_tflops"], 6.0)}
-    r64 = roof(prof64, "transform_fp64", PEAK["fp64_mfma_tflops"])
+    roofs = {"fp32": roof(res["fp32"][2], "transform_fp32", PEAK["fp32_mfma_tflops"],
+                          traf=measured_traffic("gemm_mode_kernel<float")),
+             "bf16x3": roof(res["bf16x3"][2], "transform_fp32", PEAK["bf16_mfma_tflops"], 6.0,
+                            traf=measured_traffic("gemm_bf3_kernel"))}
+    r64 = roof(prof64, "transform_fp64", PEAK["fp64_mfma_tflops"],
+               traf=measured_traffic("gemm_mode_kernel<double"))
     ms_r, cnt_r, work_r = profmx["residual_fp64"]
     res_gbs = (work_r / cnt_r) / (ms_r / cnt_r * 1e-3) / 1e9 if cnt_r else None
 
