@@ -155,6 +155,48 @@ int mf_influence_impulses(mf_solver3d *sc, const mf_der *dx, const mf_der *dy,
                           const mf_der *dz, double sigma, int face, long long first,
                           int count, double *out, void *stream);
 
+/* ---- Multidomain (influence-matrix) solve --------------------------------
+ * nd[0] x nd[1] x nd[2] subdomains sharing their interface planes
+ * (md_influence_matrix_param, src/class_md/class_md.f90:1450-1630;
+ * multidomain_solve, src/class_field/class_field.f90:1295-1388; md_solve,
+ * src/class_md/class_md.f90:379-438).  npts: the point count of every slab,
+ * x slabs then y then z (nd[0] + nd[1] + nd[2] ints, each >= 10); grids: the
+ * slab grids concatenated in the same order; consecutive slabs share their
+ * end point.  Subdomain s = a + nd[0] (b + nd[1] c) has the grids of slabs a,
+ * b, c.  bct: outer boundary codes (x1,x2,y1,y2,z1,z2; 1 Dirichlet, 2
+ * Neumann); interfaces carry Dirichlet data.  Interfaces are numbered as
+ * the reference does (x-subdomain index outermost, then y, then z; the x-,
+ * y-, z-interface above each subdomain in turn), their unknowns are the
+ * interior points of the interface plane, column-major.  Create builds one
+ * solver and three solver-mode derivative handles per subdomain and
+ * assembles the influence matrix (md_influence_matrix_set) on the device, as
+ * one dense block per subdomain over the unknowns of its interfaces; sigma
+ * is fixed at creation (the matrix depends on it).  Synchronous. */
+typedef struct mf_md mf_md;
+typedef struct mf_md_report {
+  int iterations;   /* GMRES iterations                                  */
+  double residual;  /* final ||rhs - M x||_2 / ||rhs||_2                 */
+} mf_md_report;
+int mf_md_create(const int nd[3], const int *npts, const double *grids, const int bct[6],
+                 int so, double sigma, mf_md **out);
+int mf_md_destroy(mf_md *h);
+/* Subdomain count, interface unknowns and bytes of the influence blocks. */
+int mf_md_info(const mf_md *h, int *nsub, long long *ninf, long long *matrix_bytes);
+/* y = M x for device vectors of ninf interface unknowns (the influence
+ * matrix, KSP's operator). */
+int mf_md_matvec(mf_md *h, const double *x, double *y, void *stream);
+/* Solve: g[s], u[s] are device full-size fields of subdomain s; bc[s] its
+ * boundary arrays [bcx(n2,n3,2) | bcy(n1,n3,2) | bcz(n1,n2,2)] (outer faces
+ * used, interface faces ignored).  The interface system is solved by
+ * restarted GMRES(30) from x = 0 until the relative residual is <= tol
+ * (<= 0 selects 1e-10) or max_iter iterations (<= 0: 1000); then every
+ * subdomain is solved with the interface values, FP64.  x_iface (optional,
+ * device, ninf doubles) receives the interface values.  Returns
+ * MF_ERR_NOCONV (with u written) when GMRES stops above 10 tol.
+ * Synchronises the stream. */
+int mf_md_solve(mf_md *h, const double *const *g, double *const *u, const double *const *bc,
+                double tol, int max_iter, double *x_iface, mf_md_report *rep, void *stream);
+
 /* Kernel timing (no counterpart in the reference; its solve_3d has
  * commented-out cpu_time probes, class_solver_3d.f90:106-259).  While
  * enabled, every launch of the solve is bracketed by HIP events on its

@@ -116,12 +116,22 @@ __global__ void __launch_bounds__(256) wsum_kernel(const double *__restrict__ W,
 // flow-rate state s = {g, debit, initialised, g0 (<= 0: 3 debit)}:
 // q = sum(part)/vol; first call: debit = q (1 if q <= 1e-10), g = g0;
 // then g -= (q - debit) fac1.
-__global__ void flow_kernel(const double *__restrict__ part, int np, double vol, double fac1,
-                            double *__restrict__ s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double sum = 0.0;
-  for (int b = 0; b < np; ++b) sum += part[b];
-  const double q = sum / vol;
+// One block of 1024 threads; the partials are summed in a fixed tree order
+// (deterministic).
+__global__ void __launch_bounds__(1024) flow_kernel(const double *__restrict__ part, int np,
+                                                    double vol, double fac1,
+                                                    double *__restrict__ s) {
+  __shared__ double sh[1024];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < np; b += 1024) v += part[b];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double q = sh[0] / vol;
   if (s[2] == 0.0) {
     s[1] = q > 1e-10 ? q : 1.0;
     s[0] = s[3] > 0.0 ? s[3] : 3.0 * s[1];
@@ -431,7 +441,7 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
     hipLaunchKernelGGL(wsum_kernel, dim3(s->nparts), dim3(256), 0, st, (const double *)s->W,
                        (const double *)s->q[0][L0], (const double *)s->q[0][L1],
                        (const double *)s->q[0][L2], cv[0], cv[1], cv[2], N, s->part, s->d);
-    hipLaunchKernelGGL(flow_kernel, dim3(1), dim3(1), 0, st, (const double *)s->part, s->nparts,
+    hipLaunchKernelGGL(flow_kernel, dim3(1), dim3(1024), 0, st, (const double *)s->part, s->nparts,
                        s->vol, fac1, s->fstate);
   }
   // boundary values of level it(1) with the tangential pressure gradient

@@ -418,3 +418,73 @@ class NavierStokes:
         t, g = C.c_double(), C.c_double()
         check(self._lib.mf_ns_info(self._h, C.byref(t), C.byref(g), _stream(stream)))
         return t.value, g.value
+
+
+# ---------------------------------------------------------------------------
+# Multidomain (influence-matrix) solve
+# ---------------------------------------------------------------------------
+
+class MDReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double)]
+
+
+class Multidomain:
+    """Influence-matrix multidomain solve (multidomain_solve,
+    class_field.f90:1295-1388; md_influence_matrix_set :1396-1510; md_solve,
+    class_md.f90:379-438); see mf_md_* in include/mflops/mf.h.
+
+    grids[ax][k] is the 1-D grid of slab k along axis ax (consecutive slabs
+    share their end point).  Subdomains are ordered x fastest; solve() takes
+    and returns per-subdomain lists of full-size Fortran-order device
+    fields; bc[s] is a device vector [bcx | bcy | bcz] (each Fortran-ordered)
+    of subdomain s, outer faces used."""
+
+    def __init__(self, grids, sigma: float, bct=(1,) * 6, so: int = 6):
+        lib = load()
+        self._lib = lib
+        self.nd = tuple(len(g) for g in grids)
+        flat = [np.ascontiguousarray(g, dtype=np.float64) for ax in range(3) for g in grids[ax]]
+        npts = np.array([len(g) for g in flat], dtype=np.int32)
+        allg = np.ascontiguousarray(np.concatenate(flat))
+        nd = np.array(self.nd, dtype=np.int32)
+        bc = np.array(bct, dtype=np.int32)
+        self.shapes = [tuple(len(grids[ax][s[ax]]) for ax in range(3))
+                       for s in [(a, b, c) for c in range(self.nd[2]) for b in range(self.nd[1])
+                                 for a in range(self.nd[0])]]
+        h = C.c_void_p()
+        check(lib.mf_md_create(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc), int(so),
+                               float(sigma), C.byref(h)))
+        self._h = h
+        nsub, ninf, nbytes = C.c_int(), C.c_longlong(), C.c_longlong()
+        check(lib.mf_md_info(h, C.byref(nsub), C.byref(ninf), C.byref(nbytes)))
+        self.nsub, self.ninf, self.matrix_bytes = nsub.value, ninf.value, nbytes.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mf_md_destroy(h)
+            self._h = None
+
+    def matvec(self, x, y=None, stream=None):
+        torch = _torch()
+        if y is None:
+            y = torch.empty_like(x)
+        check(self._lib.mf_md_matvec(self._h, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(y)),
+                                     _stream(stream)))
+        return y
+
+    def solve(self, g, bc, u=None, tol: float = 1e-10, max_iter: int = 0, x_iface=None,
+              stream=None):
+        if len(g) != self.nsub or len(bc) != self.nsub:
+            raise ValueError("one g and one bc per subdomain (%d)" % self.nsub)
+        if u is None:
+            u = [fortran_zeros(sh, g[0].device) for sh in self.shapes]
+        arr = C.c_void_p * self.nsub
+        gp = arr(*[_ptr(t) for t in g])
+        up = arr(*[_ptr(t) for t in u])
+        bp = arr(*[_ptr(t) for t in bc])
+        rep = MDReport()
+        xp = C.c_void_p(_ptr(x_iface)) if x_iface is not None else C.c_void_p()
+        check(self._lib.mf_md_solve(self._h, gp, up, bp, float(tol), int(max_iter), xp,
+                                    C.byref(rep), _stream(stream)))
+        return u, rep

@@ -42,6 +42,11 @@ SIGNATURES = {
     "mf_solver3d_dims": (C.c_int, [_vp, _ip, _ip, _ip]),
     "mf_influence_impulses": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_longlong,
                                         C.c_int, _vp, _vp]),
+    "mf_md_create": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.POINTER(_vp)]),
+    "mf_md_destroy": (C.c_int, [_vp]),
+    "mf_md_info": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mf_md_matvec": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mf_md_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp]),
     "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
     "mf_solver3d_profile_read": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
     "mf_ns_create": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_double, C.c_double,

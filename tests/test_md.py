@@ -1,0 +1,163 @@
+"""Multidomain (influence-matrix) solve: oracle pins (CPU) and GPU parity.
+
+The oracle (oracle/md_oracle.py) is pinned by what the mathematics fixes:
+* a polynomial that the compact schemes differentiate exactly (degree 3 in
+  each variable; every row is exact to degree 5 or more) is reproduced on
+  every subdomain: any error in the interface numbering, the sign rules of
+  md_influence_matrix_set / multidomain_solve or the assembly would show;
+* after the solve the flux jumps (sum over both sides of the outward-normal
+  first derivatives) vanish, for a non-polynomial solution too;
+* with one subdomain there are no interfaces and the solve is the plain one.
+The GPU solve (mf_md_*) is compared with the oracle on the same layouts: the
+influence matrix through M x, the interface values and every subdomain
+field, to 1e-9 relative (GMRES stops at 1e-12).
+"""
+import numpy as np
+import pytest
+
+import mf_oracle as o
+import md_oracle as md
+
+
+def layout_small():
+    # 2 x 2 x 1 subdomains, uneven point counts, one stretched slab
+    return [[o.grid_uniform(10, 0, 0.5), o.grid_uniform(12, 0.5, 1.0)],
+            [o.grid_uniform(11, 0, 0.4), o.grid_asin(10, 0.4, 1.0)],
+            [o.grid_uniform(10, 0, 1)]]
+
+
+def layout_3d():
+    # 2 x 1 x 2 subdomains with interfaces along x and z
+    return [[o.grid_uniform(11, 0, 0.3), o.grid_asin(12, 0.3, 1.0)],
+            [o.grid_uniform(10, 0, 1)],
+            [o.grid_uniform(10, 0, 0.6), o.grid_uniform(11, 0.6, 1.0)]]
+
+
+def poly(X, Y, Z):
+    return X ** 3 + 2 * Y ** 3 - Z ** 2 + X * Y * Z + 0.5 * X * Y
+
+
+def poly_lap(X, Y, Z):
+    return 6 * X + 12 * Y - 2
+
+
+def sines(X, Y, Z):
+    return np.sin(np.pi * X) * np.cos(0.5 * np.pi * Y) * np.sin(np.pi * Z + 0.3)
+
+
+def sines_lap(X, Y, Z):
+    return -(np.pi ** 2 + 0.25 * np.pi ** 2 + np.pi ** 2) * sines(X, Y, Z)
+
+
+def _exact_err(L, u, fun):
+    err = 0.0
+    for s in L.subs:
+        gx, gy, gz = (np.asarray(L.grids[ax][s[ax]]) for ax in range(3))
+        X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+        err = max(err, np.abs(u[s] - fun(X, Y, Z))[1:-1, 1:-1, 1:-1].max())
+    return err
+
+
+@pytest.mark.parametrize("lay,sigma", [(layout_small, 2.0), (layout_3d, 0.0), (layout_3d, -30.0)])
+def test_oracle_reproduces_polynomial(lay, sigma):
+    L = md.Layout(lay())
+    mdo = md.Multidomain(L, sigma)
+    g, bc = md.manufactured(L, poly, poly_lap, sigma)
+    x, u = mdo.solve(g, bc)
+    err = _exact_err(L, u, poly)
+    print("ninf %d  max |u - poly| %.2e" % (L.ninf, err))
+    assert err < 1e-10
+
+
+def test_oracle_flux_jumps_vanish():
+    L = md.Layout(layout_small())
+    sigma = 5.0
+    mdo = md.Multidomain(L, sigma)
+    g, bc = md.manufactured(L, sines, sines_lap, sigma)
+    x, u = mdo.solve(g, bc)
+    r = mdo.rhs(g, bc)
+    j = mdo.flux_jumps(u)
+    err = _exact_err(L, u, sines)
+    print("flux jumps %.2e (rhs %.2e), discretisation error %.2e" % (np.abs(j).max(), np.abs(r).max(), err))
+    assert np.abs(j).max() < 1e-10 * np.abs(r).max()
+    assert err < 1e-3  # discretisation level on 10-12 points per slab
+    # the solve with zero interface values leaves jumps of the size of rhs
+    u0 = {}
+    for s in L.subs:
+        u0[s] = np.zeros(L.shape(s), order="F")
+        o.solver_3d(mdo.sc[s], g[s], u0[s], *mdo._bc_with(s, bc[s], None), sigma)
+    assert np.abs(mdo.flux_jumps(u0) - r).max() < 1e-12 * np.abs(r).max()
+
+
+def test_oracle_single_domain_is_plain_solve():
+    grids = [[o.grid_uniform(12)], [o.grid_asin(11, 0, 1)], [o.grid_uniform(10)]]
+    L = md.Layout(grids)
+    assert L.ninf == 0
+    mdo = md.Multidomain(L, 3.0)
+    g, bc = md.manufactured(L, sines, sines_lap, 3.0)
+    _, u = mdo.solve(g, bc)
+    s = (0, 0, 0)
+    sc = o.SolverCoeffs3D(*[grids[a][0] for a in range(3)], (1,) * 6, 6)
+    ref = np.zeros(L.shape(s), order="F")
+    o.solver_3d(sc, g[s], ref, *bc[s], 3.0)
+    assert np.abs(u[s] - ref).max() == 0.0
+
+
+# ------------------------------------------------------------------- GPU ----
+
+def _mf():
+    import mflops
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    mflops.load()
+    return mflops, torch
+
+
+def _bc_vec(bc):
+    return np.concatenate([b.ravel(order="F") for b in bc])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lay,sigma,fun,lap", [(layout_small, 2.0, poly, poly_lap),
+                                               (layout_3d, -30.0, sines, sines_lap)])
+def test_gpu_md_matches_oracle(lay, sigma, fun, lap):
+    mf, torch = _mf()
+    grids = lay()
+    L = md.Layout(grids)
+    mdo = md.Multidomain(L, sigma)
+    g, bc = md.manufactured(L, fun, lap, sigma)
+    xo, uo = mdo.solve(g, bc)
+    print("run md layout %s sigma %g ninf %d" % ([len(v) for v in grids], sigma, L.ninf), flush=True)
+    h = mf.Multidomain(grids, sigma)
+    assert h.ninf == L.ninf and h.nsub == len(L.subs)
+    # the influence matrix, through M x
+    rng = np.random.default_rng(7)
+    xr = rng.standard_normal(L.ninf)
+    y = mf.to_host(h.matvec(torch.tensor(xr, device="cuda")))
+    ref = mdo.M @ xr
+    e_m = np.abs(y - ref).max() / np.abs(ref).max()
+    # the solve
+    gd = [mf.to_device(g[s]) for s in L.subs]
+    bd = [torch.tensor(_bc_vec(bc[s]), device="cuda") for s in L.subs]
+    xi = torch.zeros(L.ninf, dtype=torch.float64, device="cuda")
+    u, rep = h.solve(gd, bd, tol=1e-12, x_iface=xi)
+    e_x = np.abs(mf.to_host(xi) - xo).max() / np.abs(xo).max()
+    e_u = max(np.abs(mf.to_host(u[i]) - uo[s])[1:-1, 1:-1, 1:-1].max() for i, s in enumerate(L.subs))
+    scale = max(np.abs(uo[s]).max() for s in L.subs)
+    print("M x %.2e  x %.2e  u %.2e  GMRES %d its, residual %.2e"
+          % (e_m, e_x, e_u / scale, rep.iterations, rep.residual), flush=True)
+    assert rep.residual <= 1e-11
+    assert e_m < 1e-11
+    assert e_x < 1e-9
+    assert e_u / scale < 1e-9
+    if fun is poly:
+        assert _exact_err(L, {s: mf.to_host(u[i]) for i, s in enumerate(L.subs)}, poly) < 1e-9
+
+
+@pytest.mark.gpu
+def test_gpu_md_rejects_bad_layout():
+    mf, torch = _mf()
+    grids = layout_small()
+    grids[0][1] = o.grid_uniform(12, 0.55, 1.0)  # slabs do not touch
+    with pytest.raises(mf.MFError):
+        mf.Multidomain(grids, 1.0)
