@@ -65,7 +65,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-solves", type=int, default=2)
-    ap.add_argument("--workload", choices=("helmholtz", "influence"), default="helmholtz")
+    ap.add_argument("--workload", choices=("helmholtz", "influence", "md"), default="helmholtz")
     ap.add_argument("--ns-steps", type=int, default=10, help="timed NS steps (0: skip)")
     ap.add_argument("--ns-warmup", type=int, default=3)
     ap.add_argument("--ns-shape", type=int, nargs=3, default=(192, 256, 192))
@@ -79,6 +79,8 @@ def main():
         sys.exit(2)
     if args.workload == "influence":
         return run_influence(args)
+    if args.workload == "md":
+        return run_md(args)
 
     import numpy as np
     import torch
@@ -307,6 +309,100 @@ def cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx):
     base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": cores, "kind": "port",
             "sample": "%d full %d^3 FP64 solves (numpy/OpenBLAS dgemm)" % (args.cpu_solves, n)}
     return base, errs
+
+
+def run_md(args):
+    """Config 3 end to end: the 4x4x2 multidomain layout of run_influence
+    (slab sizes from {48,64,96}, seed 1, asin grids, Dirichlet outer faces),
+    sigma of the NS velocity solve.  Times the device assembly of the
+    influence matrix (every unit-impulse response, scattered into one dense
+    block per subdomain) and then multidomain solves of a manufactured
+    problem (zero-interface solves, GMRES on the interface system to 1e-10,
+    final solves); reports solves/s, GMRES iterations, the block-matvec HBM
+    rate and the error against the manufactured solution."""
+    import numpy as np
+    import torch
+
+    import mflops as mf
+    from mflops import synthetic
+
+    torch.cuda.set_device(0)
+    sigma = -3000.0 * 1.5 / 1e-3
+    slab, subs = synthetic.subdomain_layout()
+    nd = tuple(len(v) for v in slab)
+    grids = [[None] * nd[ax] for ax in range(3)]
+    for sd in subs:
+        for ax in range(3):
+            grids[ax][sd["index"][ax]] = sd["grids"][ax]
+    log("md: slab sizes x=%s y=%s z=%s" % tuple(slab))
+    t0 = time.perf_counter()
+    h = mf.Multidomain(grids, sigma)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    log("md: influence matrix assembled (%d unknowns, %.1f GB of blocks) in %.2f s"
+        % (h.ninf, h.matrix_bytes / 1e9, t_asm))
+
+    def f(X, Y, Z):
+        return np.sin(np.pi * X) * np.sin(2 * np.pi * Y) * np.cos(np.pi * Z)
+
+    lapc = -(np.pi ** 2) * (1 + 4 + 1)
+    order = [(a, b, c) for c in range(nd[2]) for b in range(nd[1]) for a in range(nd[0])]
+    gd, bd, exact = [], [], []
+    for s in order:
+        gx, gy, gz = (grids[ax][s[ax]] for ax in range(3))
+        X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+        U = f(X, Y, Z)
+        gd.append(mf.to_device(np.asfortranarray((lapc + sigma) * U)))
+        bc = [np.stack([U[0, 1:-1, 1:-1], U[-1, 1:-1, 1:-1]], -1),
+              np.stack([U[1:-1, 0, 1:-1], U[1:-1, -1, 1:-1]], -1),
+              np.stack([U[1:-1, 1:-1, 0], U[1:-1, 1:-1, -1]], -1)]
+        bd.append(torch.tensor(np.concatenate([b.ravel(order="F") for b in bc]), device="cuda"))
+        exact.append(U)
+    u = [mf.fortran_zeros(g.shape) for g in gd]
+    # matvec rate (the GMRES inner product is one pass over all blocks)
+    x = torch.ones(h.ninf, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    h.matvec(x, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        h.matvec(x, y)
+    torch.cuda.synchronize()
+    t_mv = (time.perf_counter() - t0) / 3
+    log("md: matvec %.2f ms (%.0f GB/s over the blocks)" % (t_mv * 1e3, h.matrix_bytes / t_mv / 1e9))
+    _, rep = h.solve(gd, bd, u=u, tol=1e-10)  # warm
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, rep = h.solve(gd, bd, u=u, tol=1e-10)
+    torch.cuda.synchronize()
+    t_solve = (time.perf_counter() - t0) / steps
+    err = max(float(np.abs(mf.to_host(ui) - e)[1:-1, 1:-1, 1:-1].max()) for ui, e in zip(u, exact))
+    log("md: solve %.3f s, GMRES %d iterations, residual %.2e, max error %.2e"
+        % (t_solve, rep.iterations, rep.residual, err))
+    out = {
+        "metric": "multidomain (influence-matrix) solves/s (config 3)",
+        "value": round(1.0 / t_solve, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "md-4x4x2-{48,64,96}-seed1", "slab_sizes": slab,
+                   "subdomains": len(order), "sigma": sigma, "tol": 1e-10,
+                   "rhs": "manufactured sin(pi x) sin(2 pi y) cos(pi z)"},
+        "ms_per_step": round(t_solve * 1e3, 2),
+        "interface_unknowns": h.ninf,
+        "influence_blocks_gb": round(h.matrix_bytes / 1e9, 2),
+        "assembly_s": round(t_asm, 2),
+        "gmres_iterations": rep.iterations,
+        "gmres_residual": rep.residual,
+        "matvec_ms": round(t_mv * 1e3, 3),
+        "matvec_gbs": round(h.matrix_bytes / t_mv / 1e9, 1),
+        "max_error_vs_manufactured": err,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_influence(args):
