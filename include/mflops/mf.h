@@ -196,6 +196,22 @@ int mf_md_matvec(mf_md *h, const double *x, double *y, void *stream);
  * Synchronises the stream. */
 int mf_md_solve(mf_md *h, const double *const *g, double *const *u, const double *const *bc,
                 double tol, int max_iter, double *x_iface, mf_md_report *rep, void *stream);
+/* Initial guess of the interface solve (md_solve with inf_sol,
+ * src/class_md/class_md.f90:389-424): when enabled, every solve starts GMRES
+ * from s1 - 3 s2 + 3 s3 of the last three interface solutions (oldest s1;
+ * zero before there are three) and then shifts its own solution in.
+ * Enabling or disabling clears the history. */
+int mf_md_set_guess(mf_md *h, int enable);
+/* Influence-matrix cache (the role of md_influence_matrix_write/read,
+ * src/class_md/class_md.f90:950-985, a PETSc binary file there): a binary
+ * file of this library's own format, a header with the layout's identity
+ * (nd, bct, so, sigma, a hash of the slab sizes and grids, sizes) then the
+ * dense blocks.  mf_md_create_cached reads the blocks from path when its
+ * header matches the arguments, else assembles them as mf_md_create and
+ * writes path; *loaded (optional) tells which. */
+int mf_md_write_matrix(const mf_md *h, const char *path);
+int mf_md_create_cached(const int nd[3], const int *npts, const double *grids, const int bct[6],
+                        int so, double sigma, const char *path, int *loaded, mf_md **out);
 
 /* Kernel timing (no counterpart in the reference; its solve_3d has
  * commented-out cpu_time probes, class_solver_3d.f90:106-259).  While

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(256) dots_sum_kernel(const double *__restrict_
   if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
-// w -= sum_j V[:, j] h[j] (j = 0..k-1); y = w * scale when out != null
+// w -= sum_j V[:, j] h[j] (j = 0..k-1)
 __global__ void __launch_bounds__(256) orth_kernel(const double *__restrict__ V, long n, int k,
                                                    const double *__restrict__ h,
                                                    double *__restrict__ w) {
@@ -151,9 +152,9 @@ __global__ void __launch_bounds__(256) orth_kernel(const double *__restrict__ V,
   w[t] = v;
 }
 
-// y = a x + b y
-__global__ void __launch_bounds__(256) axpby_kernel(double a, const double *__restrict__ x,
-                                                    double b, double *__restrict__ y, long n) {
+// y = a x + b y (x may be y)
+__global__ void __launch_bounds__(256) axpby_kernel(double a, const double *x, double b, double *y,
+                                                    long n) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t < n) y[t] = fma(a, x[t], b * y[t]);
 }
@@ -239,6 +240,9 @@ __global__ void __launch_bounds__(256) bc_iface_kernel(double *__restrict__ bc,
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
+// set while mf_md_create_cached builds a handle whose blocks it reads from a file
+thread_local bool skip_assembly = false;
+
 }  // namespace md
 }  // namespace mf
 
@@ -272,6 +276,14 @@ struct mf_md {
   double *ws = nullptr;    // GMRES and GEMV workspace
   size_t ws_doubles = 0;
   int restart = 30;
+  // initial-guess history (md_solve with inf_sol, class_md.f90:389-400,
+  // 419-424): the last three interface solutions, oldest first
+  bool guess = false;
+  double *hist = nullptr;  // 3 x ninf
+  // identity of the layout for the matrix cache
+  std::vector<int> npts;
+  std::vector<double> grids;
+  int so = 6;
   ~mf_md() {
     for (auto &s : subs) {
       if (s.sc) mf_solver3d_destroy(s.sc);
@@ -285,6 +297,7 @@ struct mf_md {
     if (pa) (void)hipFree(pa);
     if (pb) (void)hipFree(pb);
     if (ws) (void)hipFree(ws);
+    if (hist) (void)hipFree(hist);
   }
 };
 
@@ -424,6 +437,16 @@ extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grid
           return mf::fail(MF_ERR_ARG, "mf_md_create: consecutive slabs must share their end points");
   }
   mf_md *h = new mf_md();
+  h->so = so;
+  {
+    long q = 0, gofs = 0;
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < nd[a]; ++k) {
+        h->npts.push_back(npts[q]);
+        gofs += npts[q++];
+      }
+    h->grids.assign(grids, grids + gofs);
+  }
   for (int a = 0; a < 3; ++a) h->nd[a] = nd[a];
   for (int f = 0; f < 6; ++f) h->bct[f] = bct[f];
   h->sigma = sigma;
@@ -535,6 +558,13 @@ extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grid
     if (e == hipSuccess) e = hipMemcpy(s.drow, drow.data(), drow.size() * sizeof(double), hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) return bail(mf::fail(MF_ERR_DEVICE, std::string("mf_md_create: ") + hipGetErrorString(e)));
+  if (e == hipSuccess && h->ninf) e = hipMalloc(&h->hist, 3 * (size_t)h->ninf * sizeof(double));
+  if (e == hipSuccess && h->ninf) e = hipMemset(h->hist, 0, 3 * (size_t)h->ninf * sizeof(double));
+  if (e != hipSuccess) return bail(mf::fail(MF_ERR_DEVICE, std::string("mf_md_create: ") + hipGetErrorString(e)));
+  if (mf::md::skip_assembly) {  // the caller fills the blocks (matrix cache)
+    *out = h;
+    return MF_OK;
+  }
   // influence blocks from the batched unit-impulse responses
   {
     const long chunk = 1024;
@@ -638,8 +668,18 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
   hipLaunchKernelGGL(mf::md::combine_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st,
                      (const double *)w.ys, (const long *)h->pa, (const long *)h->pb, n, w.r);
   MDCHK(hipGetLastError());
-  // 2. GMRES(m) on M x = rhs, x0 = 0
-  MDCHK(hipMemsetAsync(w.x, 0, n * sizeof(double), st));
+  // 2. GMRES(m) on M x = rhs, from x0 = 0 or, with the guess history on,
+  //    x0 = s1 - 3 s2 + 3 s3 (class_md.f90:395-397)
+  if (h->guess) {
+    const double *s1 = h->hist, *s2 = s1 + n, *s3 = s2 + n;
+    MDCHK(hipMemcpyAsync(w.x, s3, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(mf::md::axpby_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st, -3.0, s2, 3.0,
+                       w.x, n);
+    hipLaunchKernelGGL(mf::md::axpby_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st, 1.0, s1, 1.0,
+                       w.x, n);
+  } else {
+    MDCHK(hipMemsetAsync(w.x, 0, n * sizeof(double), st));
+  }
   double bnorm2 = 0.0;
   MDRC(dots(h, w, w.r, 1, w.r, &bnorm2, st));
   const double bnorm = std::sqrt(bnorm2);
@@ -725,8 +765,118 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
   const double res = bnorm > 0.0 ? std::sqrt(rn2) / bnorm : 0.0;
   if (rep) { rep->iterations = iters; rep->residual = res; }
   if (x_iface) MDCHK(hipMemcpyAsync(x_iface, w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (h->guess) {  // sol_old(1) <- sol_old(2) <- sol_old(3) <- x (:419-424)
+    MDCHK(hipMemcpyAsync(h->hist, h->hist + n, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    MDCHK(hipMemcpyAsync(h->hist + 2 * n, w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
   // 3. final subdomain solves with the interface values
   MDRC(solve_subs(h, g, u, bc, w.x, st));
   if (res > tol * 10.0) return mf::fail(MF_ERR_NOCONV, "mf_md_solve: GMRES did not reach tol");
+  return MF_OK;
+}
+
+extern "C" int mf_md_set_guess(mf_md *h, int enable) {
+  if (!h) return mf::fail(MF_ERR_ARG, "mf_md_set_guess: null handle");
+  h->guess = enable != 0;
+  if (h->hist) MDCHK(hipMemset(h->hist, 0, 3 * (size_t)h->ninf * sizeof(double)));
+  return MF_OK;
+}
+
+// ---- influence-matrix cache (the role of md_influence_matrix_write/read,
+// class_md.f90:950-985, which use PETSc's binary viewer): a file of its own
+// format holding the layout's identity and the dense blocks.
+namespace {
+struct CacheHeader {
+  char magic[8];        // "MFMDBLK1"
+  int nd[3], bct[6], so, nslab;
+  double sigma;
+  unsigned long long grid_hash, ninf, doubles;
+};
+
+unsigned long long fnv1a(const void *p, size_t n, unsigned long long h = 1469598103934665603ull) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+CacheHeader header_of(const mf_md *h) {
+  CacheHeader c;
+  std::memset(&c, 0, sizeof(c));
+  std::memcpy(c.magic, "MFMDBLK1", 8);
+  for (int a = 0; a < 3; ++a) c.nd[a] = h->nd[a];
+  for (int f = 0; f < 6; ++f) c.bct[f] = h->bct[f];
+  c.so = h->so;
+  c.nslab = (int)h->npts.size();
+  c.sigma = h->sigma;
+  c.grid_hash = fnv1a(h->npts.data(), h->npts.size() * sizeof(int));
+  c.grid_hash = fnv1a(h->grids.data(), h->grids.size() * sizeof(double), c.grid_hash);
+  c.ninf = (unsigned long long)h->ninf;
+  c.doubles = (unsigned long long)h->D_doubles;
+  return c;
+}
+
+constexpr size_t kCacheChunk = 32u << 20;  // doubles per transfer
+}  // namespace
+
+extern "C" int mf_md_write_matrix(const mf_md *h, const char *path) {
+  if (!h || !path) return mf::fail(MF_ERR_ARG, "mf_md_write_matrix: null pointer");
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return mf::fail(MF_ERR_ARG, std::string("mf_md_write_matrix: cannot open ") + path);
+  const CacheHeader c = header_of(h);
+  bool ok = std::fwrite(&c, sizeof(c), 1, f) == 1;
+  std::vector<double> buf;
+  for (size_t o = 0; ok && o < h->D_doubles; o += kCacheChunk) {
+    const size_t cnt = std::min(kCacheChunk, h->D_doubles - o);
+    buf.resize(cnt);
+    ok = hipMemcpy(buf.data(), h->D + o, cnt * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+         std::fwrite(buf.data(), sizeof(double), cnt, f) == cnt;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return mf::fail(MF_ERR_DEVICE, std::string("mf_md_write_matrix: write failed: ") + path);
+  return MF_OK;
+}
+
+extern "C" int mf_md_create_cached(const int nd[3], const int *npts, const double *grids,
+                                   const int bct[6], int so, double sigma, const char *path,
+                                   int *loaded, mf_md **out) {
+  if (!path || !out) return mf::fail(MF_ERR_ARG, "mf_md_create_cached: null pointer");
+  if (loaded) *loaded = 0;
+  mf::md::skip_assembly = true;
+  mf_md *h = nullptr;
+  int rc = mf_md_create(nd, npts, grids, bct, so, sigma, &h);
+  mf::md::skip_assembly = false;
+  if (rc != MF_OK) return rc;
+  bool hit = false;
+  if (FILE *f = std::fopen(path, "rb")) {
+    CacheHeader c;
+    const CacheHeader want = header_of(h);
+    if (std::fread(&c, sizeof(c), 1, f) == 1 && std::memcmp(&c, &want, sizeof(c)) == 0) {
+      hit = true;
+      std::vector<double> buf;
+      for (size_t o = 0; hit && o < h->D_doubles; o += kCacheChunk) {
+        const size_t cnt = std::min(kCacheChunk, h->D_doubles - o);
+        buf.resize(cnt);
+        hit = std::fread(buf.data(), sizeof(double), cnt, f) == cnt &&
+              hipMemcpy(h->D + o, buf.data(), cnt * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+      }
+    }
+    std::fclose(f);
+  }
+  if (hit) {
+    if (loaded) *loaded = 1;
+    *out = h;
+    return MF_OK;
+  }
+  // no usable cache: assemble and write it
+  mf_md_destroy(h);
+  h = nullptr;
+  rc = mf_md_create(nd, npts, grids, bct, so, sigma, &h);
+  if (rc != MF_OK) return rc;
+  rc = mf_md_write_matrix(h, path);
+  if (rc != MF_OK) {
+    mf_md_destroy(h);
+    return rc;
+  }
+  *out = h;
   return MF_OK;
 }

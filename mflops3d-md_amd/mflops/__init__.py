@@ -439,7 +439,9 @@ class Multidomain:
     fields; bc[s] is a device vector [bcx | bcy | bcz] (each Fortran-ordered)
     of subdomain s, outer faces used."""
 
-    def __init__(self, grids, sigma: float, bct=(1,) * 6, so: int = 6):
+    def __init__(self, grids, sigma: float, bct=(1,) * 6, so: int = 6, cache=None):
+        """cache: optional path of the influence-matrix file (read when it
+        matches this layout, else written after assembly; self.loaded tells)."""
         lib = load()
         self._lib = lib
         self.nd = tuple(len(g) for g in grids)
@@ -452,8 +454,16 @@ class Multidomain:
                        for s in [(a, b, c) for c in range(self.nd[2]) for b in range(self.nd[1])
                                  for a in range(self.nd[0])]]
         h = C.c_void_p()
-        check(lib.mf_md_create(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc), int(so),
-                               float(sigma), C.byref(h)))
+        self.loaded = False
+        if cache is None:
+            check(lib.mf_md_create(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc), int(so),
+                                   float(sigma), C.byref(h)))
+        else:
+            ld = C.c_int()
+            check(lib.mf_md_create_cached(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc),
+                                          int(so), float(sigma), str(cache).encode(),
+                                          C.byref(ld), C.byref(h)))
+            self.loaded = bool(ld.value)
         self._h = h
         nsub, ninf, nbytes = C.c_int(), C.c_longlong(), C.c_longlong()
         check(lib.mf_md_info(h, C.byref(nsub), C.byref(ninf), C.byref(nbytes)))
@@ -464,6 +474,12 @@ class Multidomain:
         if h is not None and h.value:
             self._lib.mf_md_destroy(h)
             self._h = None
+
+    def set_guess(self, enable: bool = True):
+        check(self._lib.mf_md_set_guess(self._h, int(bool(enable))))
+
+    def write_matrix(self, path):
+        check(self._lib.mf_md_write_matrix(self._h, str(path).encode()))
 
     def matvec(self, x, y=None, stream=None):
         torch = _torch()

@@ -47,6 +47,10 @@ SIGNATURES = {
     "mf_md_info": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mf_md_matvec": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mf_md_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp]),
+    "mf_md_set_guess": (C.c_int, [_vp, C.c_int]),
+    "mf_md_write_matrix": (C.c_int, [_vp, C.c_char_p]),
+    "mf_md_create_cached": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_char_p,
+                                      C.POINTER(C.c_int), C.POINTER(_vp)]),
     "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
     "mf_solver3d_profile_read": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
     "mf_ns_create": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_double, C.c_double,

@@ -161,3 +161,44 @@ def test_gpu_md_rejects_bad_layout():
     grids[0][1] = o.grid_uniform(12, 0.55, 1.0)  # slabs do not touch
     with pytest.raises(mf.MFError):
         mf.Multidomain(grids, 1.0)
+
+
+@pytest.mark.gpu
+def test_gpu_md_matrix_cache(tmp_path):
+    mf, torch = _mf()
+    grids = layout_small()
+    path = tmp_path / "influence.bin"
+    h1 = mf.Multidomain(grids, 2.0, cache=path)
+    assert not h1.loaded and path.exists()
+    h2 = mf.Multidomain(grids, 2.0, cache=path)
+    assert h2.loaded
+    x = torch.tensor(np.random.default_rng(3).standard_normal(h1.ninf), device="cuda")
+    assert torch.equal(h1.matvec(x), h2.matvec(x))
+    h3 = mf.Multidomain(grids, 3.0, cache=path)  # another sigma: assembled again
+    assert not h3.loaded
+    assert not torch.equal(h3.matvec(x), h1.matvec(x))
+
+
+@pytest.mark.gpu
+def test_gpu_md_guess_extrapolation():
+    """s1 - 3 s2 + 3 s3 of three equal solutions is the solution itself (up
+    to 7 times their own GMRES error), so the fourth solve needs only a few
+    iterations."""
+    mf, torch = _mf()
+    grids = layout_3d()
+    L = md.Layout(grids)
+    sigma = -30.0
+    g, bc = md.manufactured(L, sines, sines_lap, sigma)
+    h = mf.Multidomain(grids, sigma)
+    h.set_guess(True)
+    gd = [mf.to_device(g[s]) for s in L.subs]
+    bd = [torch.tensor(_bc_vec(bc[s]), device="cuda") for s in L.subs]
+    its, us = [], []
+    for _ in range(4):
+        u, rep = h.solve(gd, bd, tol=1e-11)
+        its.append(rep.iterations)
+        us.append([mf.to_host(t) for t in u])
+    print("GMRES iterations with the guess history:", its, flush=True)
+    assert its[3] * 4 < its[0]
+    for a, b in zip(us[0], us[3]):
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
