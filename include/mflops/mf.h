@@ -271,6 +271,69 @@ int mf_ns_get(const mf_ns *ns, double *u, double *v, double *w, double *p, doubl
 /* Simulated time and the current uniform flow-rate force (synchronises). */
 int mf_ns_info(const mf_ns *ns, double *time, double *flow_force, void *stream);
 
+/* ---------------------------------------------------------------------
+ * Spline filter, eddy viscosities and the learned SGS closure.
+ * --------------------------------------------------------------------- */
+typedef struct mf_filter mf_filter;
+
+/* <- LSQUnivariateSpline (src/class_filter/class_filter.f90:130-174): the
+ *    least-squares spline of degree k (1..5) on the knots of curfit's
+ *    iopt = -1 call (k+1 copies of xi(1) and xi(nin), then xn(2:nn-1)),
+ *    fitted to values on xi(nin) with weights wi (curfit minimises
+ *    sum (wi (yi - s(xi)))^2; NULL = unit weights), then evaluated at
+ *    xo(nout) (deriv 0: splev, the reference's yo) or differentiated there
+ *    (deriv 1: splder, its ypo); nout <= 256.  The fit is linear in the
+ *    values: the handle holds F (nout x nin).  xi and xn increase,
+ *    xn(1) = xi(1) and xn(nn) = xi(nin); MF_ERR_SETUP when a spline has no
+ *    data point in its support (curfit's Schoenberg-Whitney condition). */
+int mf_filter_create(const double *xi, const double *wi, int nin, const double *xo, int nout,
+                     const double *xn, int nn, int k, int deriv, mf_filter **out);
+int mf_filter_destroy(mf_filter *h);
+/* F into a host array of nout*nin doubles, column-major. */
+int mf_filter_matrix(const mf_filter *h, double *F);
+/* The filter of a 3-D field, one 1-D filter per axis (the reference applies
+ * LSQUnivariateSpline line by line, ls_spline_1d.f90): in(nxi,nyi,nzi) ->
+ * out(nxo,nyo,nzo) with nxi = fx.nin, nxo = fx.nout and so on.  work holds
+ * nxo*nyi*nzi + nxo*nyo*nzi doubles.  Device pointers. */
+int mf_filter3d(const mf_filter *fx, const mf_filter *fy, const mf_filter *fz, const double *in,
+                double *out, double *work, void *stream);
+
+/* <- navier_LES (src/class_navier/class_navier_3D.f90:1225-1449), les_sigma
+ *    (:1453-1497): the eddy viscosity nu(nx,ny,nz) from the nine velocity
+ *    gradients du[3 i + j] = d u_i / d x_j (device fields), les_type
+ *    1 = Smagorinsky c sqrt(2 S:S), 2 = WALE, 3 = sigma model, times
+ *    (dx^2 + dy^2 + dz^2) / 3 (one-sided spacings on the first and last
+ *    points, centred ones inside; host grids), zero on the box edges.
+ *    c is the model constant squared (the reference's (C_s)^2 etc.).
+ *    Synchronises the stream. */
+int mf_les_nu(const double *const *du, const double *gx, int nx, const double *gy, int ny,
+              const double *gz, int nz, int les_type, double c, double *nu, void *stream);
+
+/* Learned SGS closure: MLP 10 -> 64 -> 64 -> 6 (tanh hidden layers) from
+ * (the nine filtered velocity gradients, the filter width) to the six SGS
+ * stresses (xx, yy, zz, xy, xz, yz).  5254 FP32 master parameters in the
+ * order W1 (64x10, row-major), b1, W2 (64x64), b2, W3 (6x64), b3; the
+ * kernels use bf16 copies of W1..W3 (round to nearest even) and FP32
+ * arithmetic.  No counterpart in the reference (its filter module prepares
+ * the data for such a closure, class_filter.f90). */
+typedef struct mf_mlp mf_mlp;
+int mf_mlp_create(const float *params, mf_mlp **out);  /* host params */
+int mf_mlp_destroy(mf_mlp *h);
+int mf_mlp_params(const mf_mlp *h, float *params);     /* into a host array */
+/* Y(n,6) = MLP(X(n,10)), row-major device arrays. */
+int mf_mlp_forward(const mf_mlp *h, const float *X, float *Y, long long n, void *stream);
+/* loss = mean over the n samples of sum_c (Y - T)^2 and its gradient with
+ * respect to the parameters (taken at the bf16 weights) into grad (device,
+ * 5254 floats; may be NULL).  loss (host, may be NULL) synchronises.  The
+ * per-block partial gradients are summed in a fixed order: bitwise
+ * reproducible. */
+int mf_mlp_loss_grad(mf_mlp *h, const float *X, const float *T, long long n, float *loss,
+                     float *grad, void *stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected, rate lr)
+ * on the full batch; loss (host, may be NULL) is the loss before the step. */
+int mf_mlp_adam_step(mf_mlp *h, const float *X, const float *T, long long n, float lr,
+                     float *loss, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

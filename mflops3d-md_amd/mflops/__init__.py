@@ -504,3 +504,136 @@ class Multidomain:
         check(self._lib.mf_md_solve(self._h, gp, up, bp, float(tol), int(max_iter), xp,
                                     C.byref(rep), _stream(stream)))
         return u, rep
+
+
+# ---------------------------------------------------------------------------
+# Spline filter, eddy viscosities, learned SGS closure
+# ---------------------------------------------------------------------------
+
+class SplineFilter:
+    """1-D least-squares spline filter (LSQUnivariateSpline,
+    class_filter.f90:130-174): degree-k spline on curfit's iopt = -1 knots
+    built from the nodes xn, fitted to values on xi with weights wi,
+    evaluated at xo (deriv=0) or differentiated there (deriv=1)."""
+
+    def __init__(self, xi, xo, xn, k: int = 3, wi=None, deriv: int = 0):
+        lib = load()
+        self._lib = lib
+        xi, xo, xn = (np.ascontiguousarray(a, dtype=np.float64) for a in (xi, xo, xn))
+        w = C.c_void_p()
+        if wi is not None:
+            self._wi = np.ascontiguousarray(wi, dtype=np.float64)
+            if self._wi.shape != xi.shape:
+                raise ValueError("one weight per input point")
+            w = _dptr(self._wi)
+        self.nin, self.nout = len(xi), len(xo)
+        h = C.c_void_p()
+        check(lib.mf_filter_create(_dptr(xi), w, len(xi), _dptr(xo), len(xo), _dptr(xn),
+                                   len(xn), int(k), int(deriv), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mf_filter_destroy(h)
+            self._h = None
+
+    def matrix(self) -> np.ndarray:
+        F = np.zeros((self.nout, self.nin), order="F")
+        check(self._lib.mf_filter_matrix(self._h, _dptr(F)))
+        return F
+
+
+def filter3d(fx: SplineFilter, fy: SplineFilter, fz: SplineFilter, x, out=None, stream=None):
+    """Filter a (nxi, nyi, nzi) Fortran-order device field to (nxo, nyo, nzo)."""
+    torch = _torch()
+    if tuple(x.shape) != (fx.nin, fy.nin, fz.nin):
+        raise ValueError("field shape %s != (%d, %d, %d)" % (tuple(x.shape), fx.nin, fy.nin, fz.nin))
+    if out is None:
+        out = fortran_empty((fx.nout, fy.nout, fz.nout), x.device)
+    work = torch.empty(fx.nout * fy.nin * fz.nin + fx.nout * fy.nout * fz.nin,
+                       dtype=torch.float64, device=x.device)
+    check(fx._lib.mf_filter3d(fx._h, fy._h, fz._h, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(out)),
+                              C.c_void_p(work.data_ptr()), _stream(stream)))
+    return out
+
+
+LES_SMAGORINSKY, LES_WALE, LES_SIGMA = 1, 2, 3
+
+
+def les_nu(du, gx, gy, gz, les_type: int, c: float, out=None, stream=None):
+    """Eddy viscosity (navier_LES, class_navier_3D.f90:1225-1449) from
+    du[i][j] = d u_i / d x_j (device fields) on the grids gx, gy, gz."""
+    lib = load()
+    gs = [np.ascontiguousarray(g, dtype=np.float64) for g in (gx, gy, gz)]
+    shape = tuple(len(g) for g in gs)
+    flat = [du[i][j] for i in range(3) for j in range(3)]
+    for t in flat:
+        if tuple(t.shape) != shape:
+            raise ValueError("gradient shape %s != %s" % (tuple(t.shape), shape))
+    if out is None:
+        out = fortran_empty(shape, flat[0].device)
+    arr = (C.c_void_p * 9)(*[_ptr(t) for t in flat])
+    check(lib.mf_les_nu(arr, _dptr(gs[0]), shape[0], _dptr(gs[1]), shape[1], _dptr(gs[2]),
+                        shape[2], int(les_type), float(c), C.c_void_p(_ptr(out)), _stream(stream)))
+    return out
+
+
+MLP_SIZES = (10, 64, 64, 6)
+MLP_NPARAM = sum(a * b + b for a, b in zip(MLP_SIZES[:-1], MLP_SIZES[1:]))
+
+
+class SGSNet:
+    """Learned SGS closure 10 -> 64 -> 64 -> 6 (tanh) on the device; see
+    mf_mlp_* in include/mflops/mf.h.  X (n, 10) and T (n, 6) are row-major
+    float32 device tensors."""
+
+    def __init__(self, params):
+        lib = load()
+        self._lib = lib
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != MLP_NPARAM:
+            raise ValueError("expected %d parameters, got %d" % (MLP_NPARAM, p.size))
+        h = C.c_void_p()
+        check(lib.mf_mlp_create(p.ctypes.data_as(C.c_void_p), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.mf_mlp_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def _f32(t, cols):
+        torch = _torch()
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.dim() != 2 or t.shape[1] != cols:
+            raise ValueError("expected a contiguous float32 (n, %d) tensor" % cols)
+        return C.c_void_p(t.data_ptr())
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        check(self._lib.mf_mlp_params(self._h, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def forward(self, X, out=None, stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty((X.shape[0], 6), dtype=torch.float32, device=X.device)
+        check(self._lib.mf_mlp_forward(self._h, self._f32(X, 10), self._f32(out, 6), X.shape[0],
+                                       _stream(stream)))
+        return out
+
+    def loss_grad(self, X, T, stream=None):
+        torch = _torch()
+        g = torch.empty(MLP_NPARAM, dtype=torch.float32, device=X.device)
+        loss = C.c_float()
+        check(self._lib.mf_mlp_loss_grad(self._h, self._f32(X, 10), self._f32(T, 6), X.shape[0],
+                                         C.byref(loss), C.c_void_p(g.data_ptr()), _stream(stream)))
+        return loss.value, g
+
+    def adam_step(self, X, T, lr: float = 1e-3, stream=None) -> float:
+        loss = C.c_float()
+        check(self._lib.mf_mlp_adam_step(self._h, self._f32(X, 10), self._f32(T, 6), X.shape[0],
+                                         float(lr), C.byref(loss), _stream(stream)))
+        return loss.value

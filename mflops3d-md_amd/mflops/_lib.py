@@ -63,6 +63,19 @@ SIGNATURES = {
     "mf_ns_step": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _vp]),
     "mf_ns_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mf_ns_info": (C.c_int, [_vp, _dp, _dp, _vp]),
+    "mf_filter_create": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int,
+                                   C.c_int, C.POINTER(_vp)]),
+    "mf_filter_destroy": (C.c_int, [_vp]),
+    "mf_filter_matrix": (C.c_int, [_vp, _vp]),
+    "mf_filter3d": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mf_les_nu": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_double,
+                            _vp, _vp]),
+    "mf_mlp_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "mf_mlp_destroy": (C.c_int, [_vp]),
+    "mf_mlp_params": (C.c_int, [_vp, _vp]),
+    "mf_mlp_forward": (C.c_int, [_vp, _vp, _vp, C.c_longlong, _vp]),
+    "mf_mlp_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp]),
+    "mf_mlp_adam_step": (C.c_int, [_vp, _vp, _vp, C.c_longlong, C.c_float, _vp, _vp]),
 }
 
 PROF_CLASSES = ("fold", "transform_fp64", "transform_fp32", "residual_fp64",

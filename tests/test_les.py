@@ -1,0 +1,314 @@
+"""Spline filter, eddy viscosities and the learned SGS closure.
+
+Oracle pins (CPU; oracle/les_oracle.py):
+* the filter matrix against scipy.interpolate.LSQUnivariateSpline (the same
+  FITPACK curfit/splev/splder routines the reference calls), degrees 1-5,
+  with weights and the derivative output; polynomials of degree <= k pass
+  through unchanged (and are differentiated exactly);
+* Smagorinsky on a simple shear du(1,2) = g: nu = c |g| (dx^2+dy^2+dz^2)/3;
+* WALE vanishes for a pure shear, the sigma model for two-dimensional
+  gradients and for solid rotation; the sigma model's singular values equal
+  numpy's SVD; every model is invariant under a rotation of the axes
+  (g -> R g R^T), which an index or transposition mistake breaks;
+* the MLP gradient against central finite differences (float64, unrounded
+  weights), and the bf16-rounded gradient against the unrounded one taken at
+  the rounded weights.
+GPU parity (mf_filter_*, mf_les_nu, mf_mlp_*) against the oracle on the same
+inputs: the filter matrix and the filtered field to 1e-12 relative (FP64),
+the viscosities to 1e-12 relative (sigma model 1e-9), the MLP output and gradients to the FP32
+arithmetic level (2e-5 and 1e-4 relative to the largest entry).
+"""
+import numpy as np
+import pytest
+
+import les_oracle as L
+
+
+def _grid_stretched(n, a=2.0):
+    s = np.linspace(-1.0, 1.0, n)
+    return 0.5 * (1.0 + np.tanh(a * s) / np.tanh(a))
+
+
+# ------------------------------------------------------------------ filter ---
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_oracle_filter_matches_scipy(k):
+    from scipy.interpolate import LSQUnivariateSpline
+    rng = np.random.default_rng(k)
+    xi = _grid_stretched(61)
+    xo = np.linspace(0.0, 1.0, 23)
+    xn = np.linspace(0.0, 1.0, 9)
+    w = rng.uniform(0.5, 2.0, len(xi))
+    y = rng.standard_normal(len(xi))
+    for wi in (None, w):
+        s = LSQUnivariateSpline(xi, y, xn[1:-1], w=wi, k=k)
+        F = L.lsq_filter_matrix(xi, xo, xn, k, wi)
+        D = L.lsq_filter_matrix(xi, xo, xn, k, wi, deriv=1)
+        assert np.abs(F @ y - s(xo)).max() < 1e-13 * np.abs(y).max()
+        dref = s.derivative()(xo)
+        assert np.abs(D @ y - dref).max() < 1e-12 * np.abs(dref).max()
+
+
+@pytest.mark.parametrize("k", [1, 3, 5])
+def test_oracle_filter_reproduces_polynomials(k):
+    xi = _grid_stretched(50)
+    xo = _grid_stretched(17, 1.0)
+    xn = np.linspace(0.0, 1.0, 7)
+    F = L.lsq_filter_matrix(xi, xo, xn, k)
+    D = L.lsq_filter_matrix(xi, xo, xn, k, deriv=1)
+    for deg in range(k + 1):
+        assert np.abs(F @ (xi ** deg) - xo ** deg).max() < 1e-12
+        exact = deg * xo ** (deg - 1) if deg else np.zeros_like(xo)
+        assert np.abs(D @ (xi ** deg) - exact).max() < 1e-10
+    # one degree more is not reproduced (the spline space is what it is)
+    assert np.abs(F @ (xi ** (k + 1)) - xo ** (k + 1)).max() > 1e-8
+
+
+# ------------------------------------------------------- eddy viscosities ---
+
+def _grids(n=(7, 6, 5)):
+    return (_grid_stretched(n[0]), np.linspace(0, 2, n[1]), _grid_stretched(n[2], 1.0))
+
+
+def _const_du(g, shape):
+    return [[np.full(shape, g[i, j]) for j in range(3)] for i in range(3)]
+
+
+def _delta2(gx, gy, gz):
+    d = [L._spacing(g) ** 2 for g in (gx, gy, gz)]
+    return (d[0][:, None, None] + d[1][None, :, None] + d[2][None, None, :]) / 3.0
+
+
+def _interior(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+def test_oracle_smagorinsky_simple_shear():
+    gx, gy, gz = _grids()
+    shape = (len(gx), len(gy), len(gz))
+    g = np.zeros((3, 3))
+    g[0, 1] = -2.5
+    nu = L.les_nu(_const_du(g, shape), gx, gy, gz, 1, 0.17 ** 2)
+    ref = 0.17 ** 2 * 2.5 * _delta2(gx, gy, gz)
+    assert np.abs(_interior(nu - ref)).max() < 1e-15
+    # box edges are zero, faces are not
+    assert nu[0, 0, :].max() == 0 and nu[:, -1, 0].max() == 0 and nu[0, 2, 2] > 0
+
+
+def test_oracle_wale_sigma_vanish():
+    gx, gy, gz = _grids()
+    shape = (len(gx), len(gy), len(gz))
+    shear = np.zeros((3, 3))
+    shear[1, 2] = 3.0
+    assert np.abs(L.les_nu(_const_du(shear, shape), gx, gy, gz, 2, 0.25)).max() == 0.0
+    # the sigma model takes s3 = sqrt(i1 - s1 - s2): a zero singular value
+    # comes out at the square root of the rounding, ~1e-8 of the gradient
+    g2d = np.array([[0.3, -1.2, 0.0], [0.7, -0.3, 0.0], [0.0, 0.0, 0.0]])
+    smag = np.abs(L.les_nu(_const_du(g2d, shape), gx, gy, gz, 1, 2.25)).max()
+    assert np.abs(L.les_nu(_const_du(g2d, shape), gx, gy, gz, 3, 2.25)).max() < 1e-7 * smag
+    rot = np.array([[0.0, 1.1, -0.4], [-1.1, 0.0, 0.8], [0.4, -0.8, 0.0]])
+    assert np.abs(L.les_nu(_const_du(rot, shape), gx, gy, gz, 3, 2.25)).max() < 1e-7 * smag
+    # for a general gradient none of them is zero
+    g = np.random.default_rng(0).standard_normal((3, 3))
+    for t in (1, 2, 3):
+        assert _interior(L.les_nu(_const_du(g, shape), gx, gy, gz, t, 1.0)).min() > 0
+
+
+def test_oracle_sigma_values_are_singular_values():
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((200, 3, 3))
+    s = np.stack(L.les_sigma(g), -1)
+    ref = np.linalg.svd(g, compute_uv=False)
+    assert np.abs(s - ref).max() < 1e-6 * ref.max()
+
+
+@pytest.mark.parametrize("les_type", [1, 2, 3])
+def test_oracle_models_are_rotation_invariant(les_type):
+    gx, gy, gz = _grids((5, 5, 5))
+    shape = (5, 5, 5)
+    rng = np.random.default_rng(11 + les_type)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    for _ in range(5):
+        g = rng.standard_normal((3, 3))
+        a = L.les_nu(_const_du(g, shape), gx, gy, gz, les_type, 1.0)[2, 2, 2]
+        b = L.les_nu(_const_du(q @ g @ q.T, shape), gx, gy, gz, les_type, 1.0)[2, 2, 2]
+        assert abs(a - b) < 1e-10 * abs(a)
+
+
+# ---------------------------------------------------------------- the MLP ---
+
+def _mlp_data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, 10)).astype(np.float32)
+    T = (0.3 * rng.standard_normal((n, 6))).astype(np.float32)
+    return X, T
+
+
+def test_oracle_mlp_gradient_finite_differences():
+    p = L.init_params(3).astype(np.float64)
+    p[L.NPARAM - 6:] = 0.1  # nonzero output biases
+    X, T = _mlp_data(17, 4)
+    _, g = L.loss_and_grad(p, X, T, rounding=False)
+    rng = np.random.default_rng(9)
+    idx = np.concatenate([rng.choice(L.NPARAM, 40, replace=False),
+                          [0, 639, 640, 703, 704, 4799, 4800, 4863, 4864, 5247, 5248, 5253]])
+    h = 1e-6
+    for q in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[q] += h
+        pm[q] -= h
+        fd = (L.loss_and_grad(pp, X, T, False)[0] - L.loss_and_grad(pm, X, T, False)[0]) / (2 * h)
+        assert abs(fd - g[q]) < 1e-7 + 1e-6 * abs(g[q]), (q, fd, g[q])
+
+
+def test_oracle_mlp_rounding_is_weights_only():
+    p = L.init_params(8)
+    X, T = _mlp_data(33, 2)
+    lr, gr = L.loss_and_grad(p, X, T, rounding=True)
+    pr = p.astype(np.float64).copy()
+    W1, b1, W2, b2, W3, b3 = L.unpack(pr)
+    for w in (W1, W2, W3):
+        w[...] = L.bf16(w.astype(np.float32))
+    lu, gu = L.loss_and_grad(pr, X, T, rounding=False)
+    assert abs(lr - lu) < 1e-12 * lu and np.abs(gr - gu).max() < 1e-12 * np.abs(gu).max()
+
+
+# ------------------------------------------------------------------- GPU ----
+
+def _mf():
+    import mflops
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    mflops.load()
+    return mflops, torch
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,deriv,weighted", [(3, 0, False), (5, 0, True), (3, 1, True), (1, 0, False)])
+def test_gpu_filter_matrix(k, deriv, weighted):
+    mf, torch = _mf()
+    xi = _grid_stretched(97)
+    xo = np.linspace(0, 1, 33)
+    xn = np.linspace(0, 1, 13)
+    wi = np.random.default_rng(1).uniform(0.5, 2.0, len(xi)) if weighted else None
+    print("run filter k %d deriv %d weighted %d" % (k, deriv, weighted), flush=True)
+    F = mf.SplineFilter(xi, xo, xn, k, wi, deriv).matrix()
+    ref = L.lsq_filter_matrix(xi, xo, xn, k, wi, deriv)
+    assert np.abs(F - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_gpu_filter_rejects_bad_nodes():
+    mf, torch = _mf()
+    xi = np.linspace(0, 1, 20)
+    with pytest.raises(mf.MFError):
+        mf.SplineFilter(xi, xi, np.array([0.0, 0.5, 0.9]), 3)  # last node != last point
+    with pytest.raises(mf.MFError):  # the spline on [0.5, 0.504] has no data point
+        mf.SplineFilter(xi, xi, np.array([0.0, 0.5, 0.501, 0.502, 0.503, 0.504, 1.0]), 3)
+    with pytest.raises(mf.MFError):
+        mf.SplineFilter(xi, xi, np.array([0.0, 0.6, 0.4, 1.0]), 3)  # nodes not increasing
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape_in,shape_out", [((40, 33, 29), (12, 10, 9)), ((70, 20, 130), (70, 20, 33))])
+def test_gpu_filter3d(shape_in, shape_out):
+    mf, torch = _mf()
+    rng = np.random.default_rng(sum(shape_in))
+    fs, Fs = [], []
+    for a in range(3):
+        xi = _grid_stretched(shape_in[a], 1.5)
+        xo = np.linspace(0, 1, shape_out[a])
+        xn = np.linspace(0, 1, max(2, shape_in[a] // 4))
+        fs.append(mf.SplineFilter(xi, xo, xn, 3))
+        Fs.append(L.lsq_filter_matrix(xi, xo, xn, 3))
+    x = rng.standard_normal(shape_in)
+    print("run filter3d %s -> %s" % (shape_in, shape_out), flush=True)
+    y = mf.to_host(mf.filter3d(*fs, mf.to_device(x)))
+    ref = np.einsum("ai,bj,ck,ijk->abc", Fs[0], Fs[1], Fs[2], x)
+    assert y.shape == shape_out
+    assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("les_type", [1, 2, 3])
+def test_gpu_les_nu(les_type):
+    mf, torch = _mf()
+    gx, gy, gz = _grid_stretched(21), np.linspace(0, 2, 18), _grid_stretched(15, 1.0)
+    shape = (21, 18, 15)
+    rng = np.random.default_rng(les_type)
+    du = [[rng.standard_normal(shape) for _ in range(3)] for _ in range(3)]
+    # a few exact special cases inside the random field
+    du[0][0][3, 3, 3] = du[1][1][3, 3, 3] = du[2][2][3, 3, 3] = 0.0
+    for i in range(3):
+        for j in range(3):
+            du[i][j][4, 4, 4] = 0.0
+    c = {1: 0.17 ** 2, 2: 0.5 ** 2, 3: 1.5 ** 2}[les_type]
+    ref = L.les_nu(du, gx, gy, gz, les_type, c)
+    dd = [[mf.to_device(du[i][j]) for j in range(3)] for i in range(3)]
+    print("run les_nu type %d" % les_type, flush=True)
+    nu = mf.to_host(mf.les_nu(dd, gx, gy, gz, les_type, c))
+    err = np.abs(nu - ref).max() / np.abs(ref).max()
+    print("max relative difference %.2e" % err, flush=True)
+    # the sigma model's acos of the invariant ratio amplifies rounding by
+    # 1/sqrt(1 - r^2) near r = +-1, and the device acos/pow are not libm's
+    assert err < (1e-12 if les_type < 3 else 1e-9)
+    assert nu[4, 4, 4] == 0.0 and nu[0, 0, 5] == 0.0
+
+
+@pytest.mark.gpu
+def test_gpu_mlp_forward_and_gradient():
+    mf, torch = _mf()
+    p = L.init_params(21)
+    p[L.NPARAM - 6:] = 0.05
+    X, T = _mlp_data(1000, 22)  # 15 full blocks and a partial one
+    net = mf.SGSNet(p)
+    assert np.array_equal(net.params(), p)
+    Xd, Td = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    print("run mlp forward/grad n=%d" % len(X), flush=True)
+    Y = net.forward(Xd).cpu().numpy()
+    Yr = L.forward(p, X)[0]
+    assert np.abs(Y - Yr).max() < 2e-5 * np.abs(Yr).max()
+    loss, g = net.loss_grad(Xd, Td)
+    lr, gr = L.loss_and_grad(p, X, T)
+    g = g.cpu().numpy()
+    assert abs(loss - lr) < 1e-5 * lr
+    assert np.abs(g - gr).max() < 1e-4 * np.abs(gr).max()
+    # the fixed-order reduction is bitwise reproducible
+    loss2, g2 = net.loss_grad(Xd, Td)
+    assert loss2 == loss and np.array_equal(g2.cpu().numpy(), g)
+
+
+@pytest.mark.gpu
+def test_gpu_mlp_adam_matches_oracle_and_trains():
+    """Three Adam steps track an FP64 restatement of Adam on the oracle's
+    gradients; 300 more steps fit a smooth target."""
+    mf, torch = _mf()
+    rng = np.random.default_rng(31)
+    X = rng.uniform(-1, 1, (2048, 10)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0] + X[:, 1]), X[:, 2] * X[:, 3], np.tanh(X[:, 4]),
+                  0.5 * X[:, 5], np.cos(X[:, 6]) - 0.5, X[:, 7] * X[:, 8] - X[:, 9]], 1).astype(np.float32)
+    p = L.init_params(5)
+    net = mf.SGSNet(p)
+    Xd, Td = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    lr = 2e-3
+    m = np.zeros(L.NPARAM)
+    v = np.zeros(L.NPARAM)
+    pr = p.astype(np.float64)
+    print("run mlp adam", flush=True)
+    for t in range(1, 4):
+        loss = net.adam_step(Xd, Td, lr)
+        lref, g = L.loss_and_grad(pr.astype(np.float32), X, T)
+        assert abs(loss - lref) < 1e-4 * lref
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        pr = pr - lr * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+    # a parameter with a tiny gradient may move by up to lr per step in
+    # either direction under Adam; compare where the gradient is not tiny
+    pd = net.params()
+    big = np.abs(g) > 1e-3 * np.abs(g).max()
+    assert np.abs(pd - pr)[big].max() < 1e-2 * lr
+    first = loss
+    for _ in range(300):
+        loss = net.adam_step(Xd, Td, lr)
+    print("loss %.4e -> %.4e" % (first, loss), flush=True)
+    assert loss < 0.1 * first
