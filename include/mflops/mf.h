@@ -270,6 +270,16 @@ int mf_ns_get(const mf_ns *ns, double *u, double *v, double *w, double *p, doubl
               void *stream);
 /* Simulated time and the current uniform flow-rate force (synchronises). */
 int mf_ns_info(const mf_ns *ns, double *time, double *flow_force, void *stream);
+/* <- les_type / les_c (navier_initialization :2137-2138), navier_LES
+ *    (:1225-1452) and the LES part of f(nav,var) (:1083-1134): from the next
+ *    step on, each step computes the eddy viscosity nu of model les_type
+ *    (0 = off, 1 = Smagorinsky, 2 = WALE, 3 = sigma; constant c, squared
+ *    as the reference does) from the latest level's velocity gradients and
+ *    adds sum_j d/dx_j (nu (du_i/dx_j + du_j/dx_i)) to the force of u_i. */
+int mf_ns_set_les(mf_ns *ns, int les_type, double c, void *stream);
+/* nu of the last step with LES into a device field (<- the les_nu output
+ * field, class_navier_3D.f90:101). */
+int mf_ns_les_nu(const mf_ns *ns, double *nu, void *stream);
 
 /* ---------------------------------------------------------------------
  * Spline filter, eddy viscosities and the learned SGS closure.

@@ -46,6 +46,15 @@ void derivative_rows(int n, const double *dl_lu, const double *dr, double *row0,
 bool der_host_tables(const struct ::mf_der *dc, int order, int sub, const double **dl_lu,
                      const double **dr, int *n);
 
+// Squared spacings of one grid for the eddy viscosity's filter width
+// (one-sided on the first and last points, centred inside).
+void les_spacing2(const double *g, int n, double *out);
+// Eddy-viscosity kernel launch (les.hip): du[3 i + j] = d u_i / d x_j, d2x/y/z
+// device arrays of squared spacings, stream a hipStream_t.
+void les_nu_launch(const double *const du[9], const double *d2x, const double *d2y,
+                   const double *d2z, int nx, int ny, int nz, int type, double c, double *nu,
+                   void *stream);
+
 // thread-local last error message (mf_last_error)
 int fail(int code, const std::string &msg);
 

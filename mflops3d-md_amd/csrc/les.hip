@@ -522,6 +522,25 @@ extern "C" int mf_filter3d(const mf_filter *fx, const mf_filter *fy, const mf_fi
   return MF_OK;
 }
 
+void mf::les_spacing2(const double *g, int n, double *out) {
+  for (int i = 0; i < n; ++i) {
+    const double d = (i == 0) ? g[1] - g[0] : (i == n - 1 ? g[i] - g[i - 1] : 0.5 * (g[i + 1] - g[i - 1]));
+    out[i] = d * d;
+  }
+}
+
+void mf::les_nu_launch(const double *const du[9], const double *d2x, const double *d2y,
+                       const double *d2z, int nx, int ny, int nz, int type, double c, double *nu,
+                       void *stream) {
+  mf::les::NuArgs a;
+  for (int q = 0; q < 9; ++q) a.du[q] = du[q];
+  a.d2x = d2x; a.d2y = d2y; a.d2z = d2z;
+  a.nx = nx; a.ny = ny; a.nz = nz; a.type = type; a.c = c; a.nu = nu;
+  const long N = (long)nx * ny * nz;
+  hipLaunchKernelGGL(mf::les::les_nu_kernel, dim3(mf::les::nblk(N)), dim3(256), 0,
+                     (hipStream_t)stream, a);
+}
+
 extern "C" int mf_les_nu(const double *const *du, const double *gx, int nx, const double *gy, int ny,
                          const double *gz, int nz, int les_type, double c, double *nu, void *stream) {
   if (!du || !gx || !gy || !gz || !nu) return mf::fail(MF_ERR_ARG, "mf_les_nu: null pointer");
@@ -530,25 +549,15 @@ extern "C" int mf_les_nu(const double *const *du, const double *gx, int nx, cons
   for (int q = 0; q < 9; ++q)
     if (!du[q]) return mf::fail(MF_ERR_ARG, "mf_les_nu: null gradient field");
   // squared spacings (navier_LES :1416-1446)
-  std::vector<double> d2;
-  const double *g3[3] = {gx, gy, gz};
-  const int n3[3] = {nx, ny, nz};
-  for (int a = 0; a < 3; ++a)
-    for (int i = 0; i < n3[a]; ++i) {
-      const double *g = g3[a];
-      const double d = (i == 0) ? g[1] - g[0] : (i == n3[a] - 1 ? g[i] - g[i - 1] : 0.5 * (g[i + 1] - g[i - 1]));
-      d2.push_back(d * d);
-    }
+  std::vector<double> d2((size_t)nx + ny + nz);
+  mf::les_spacing2(gx, nx, d2.data());
+  mf::les_spacing2(gy, ny, d2.data() + nx);
+  mf::les_spacing2(gz, nz, d2.data() + nx + ny);
   hipStream_t st = (hipStream_t)stream;
   double *dd = nullptr;
   LESCHK(hipMallocAsync(&dd, d2.size() * sizeof(double), st));
   LESCHK(hipMemcpyAsync(dd, d2.data(), d2.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  mf::les::NuArgs a;
-  for (int q = 0; q < 9; ++q) a.du[q] = du[q];
-  a.d2x = dd; a.d2y = dd + nx; a.d2z = dd + nx + ny;
-  a.nx = nx; a.ny = ny; a.nz = nz; a.type = les_type; a.c = c; a.nu = nu;
-  const long N = (long)nx * ny * nz;
-  hipLaunchKernelGGL(mf::les::les_nu_kernel, dim3(mf::les::nblk(N)), dim3(256), 0, st, a);
+  mf::les_nu_launch(du, dd, dd + nx, dd + nx + ny, nx, ny, nz, les_type, c, nu, st);
   LESCHK(hipGetLastError());
   LESCHK(hipFreeAsync(dd, st));
   LESCHK(hipStreamSynchronize(st));  // d2 goes out of scope

@@ -147,6 +147,7 @@ struct RhsArgs {
   double c0, c1, c2;              // extrapolation coefficients
   const double *force;            // or null
   const double *gflow;            // uniform force (device scalar) or null
+  const double *les;              // -les_f: sum_j d_j (nu (du_ij + du_ji)), or null
   double re;
   double *out;
   long N;
@@ -164,6 +165,7 @@ __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a) {
   if (edge) ne = 0.0;
   double f = a.force ? a.force[t] : 0.0;
   if (a.gflow) f += a.gflow[0];
+  if (a.les) f += a.les[t];
   double rhs = f - ne;
   if (edge) rhs = 0.0;
   double g = a.f2 * a.q0[t] + a.f3 * a.q1[t];
@@ -290,6 +292,42 @@ __global__ void __launch_bounds__(256) zero_edges_kernel(double *__restrict__ f,
 }
 
 // p = zero_edges(phi - fphi/(fac1 Re)); q_c = zero_edges((rp_c - dphi_c)/fac1)
+// aux_i = nu (du_ij + du_ji), i = 0..2, for one derivative direction j
+// (f(nav,var), class_navier_3D.f90:1090-1099)
+__global__ void __launch_bounds__(256) les_aux_kernel(const double *__restrict__ nu,
+                                                      const double *__restrict__ a0,
+                                                      const double *__restrict__ a1,
+                                                      const double *__restrict__ a2,
+                                                      const double *__restrict__ b0,
+                                                      const double *__restrict__ b1,
+                                                      const double *__restrict__ b2,
+                                                      double *__restrict__ o0,
+                                                      double *__restrict__ o1,
+                                                      double *__restrict__ o2, long N) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const double n = nu[t];
+  o0[t] = n * (a0[t] + b0[t]);
+  o1[t] = n * (a1[t] + b1[t]);
+  o2[t] = n * (a2[t] + b2[t]);
+}
+
+// acc_i (=|+=) d_i, i = 0..2
+__global__ void __launch_bounds__(256) les_acc_kernel(const double *__restrict__ d0,
+                                                      const double *__restrict__ d1,
+                                                      const double *__restrict__ d2,
+                                                      double *__restrict__ c0,
+                                                      double *__restrict__ c1,
+                                                      double *__restrict__ c2, int first, long N) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  if (first) {
+    c0[t] = d0[t]; c1[t] = d1[t]; c2[t] = d2[t];
+  } else {
+    c0[t] += d0[t]; c1[t] += d1[t]; c2[t] += d2[t];
+  }
+}
+
 __global__ void __launch_bounds__(256) proj_kernel(const double *__restrict__ phi,
                                                    const double *__restrict__ fphi,
                                                    const double *__restrict__ ru,
@@ -355,6 +393,14 @@ struct mf_ns {
   int it[NT] = {0, 1, 2, 3};
   double time = 0.0;
   mf::ns::Dims d{0, 0, 0};
+  std::vector<double> gx, gy, gz;
+  int grad_level = -1;  // level whose velocity gradients T[0..8] hold
+  // LES (navier_LES): model, constant squared, fields -les_f (3) and nu,
+  // squared spacings
+  int les_type = 0;
+  double les_c2 = 0.0;
+  void *lesmem = nullptr;
+  double *lesf[3] = {nullptr, nullptr, nullptr}, *lesnu = nullptr, *d2 = nullptr;
 };
 
 namespace {
@@ -399,6 +445,7 @@ int nonlinear(mf_ns *s, int L, hipStream_t st) {
   double *u = s->q[0][L], *v = s->q[1][L], *w = s->q[2][L];
   double **T = s->T;  // T[3a + c] gradients, T[9..14] products, T[15 + 3a + c] product derivatives
   for (int a = 0; a < 3; ++a) RC(der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
+  s->grad_level = L;
   NlArgs n;
   std::memset(&n, 0, sizeof(n));
   n.u = u; n.v = v; n.w = w;
@@ -423,6 +470,35 @@ int nonlinear(mf_ns *s, int L, hipStream_t st) {
   return MF_OK;
 }
 
+// navier_LES (:1225-1452) on level L and the LES part of f(nav,var)
+// (:1083-1134): lesf_i = -les_f_i = sum_j d_j (nu (du_ij + du_ji)).  Uses
+// T[0..8] (gradients), T[9..14] and keeps nu in lesnu.
+int les_force(mf_ns *s, int L, hipStream_t st) {
+  double **T = s->T;
+  const long N = s->N;
+  if (s->grad_level != L) {
+    for (int a = 0; a < 3; ++a)
+      RC(der3(s, a, s->q[0][L], s->q[1][L], s->q[2][L], T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
+    s->grad_level = L;
+  }
+  const double *du[9];  // du[3 i + j] = d u_i / d x_j = T[3 j + i]
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) du[3 * i + j] = T[3 * j + i];
+  mf::les_nu_launch(du, s->d2, s->d2 + s->nx, s->d2 + s->nx + s->ny, s->nx, s->ny, s->nz,
+                    s->les_type, s->les_c2, s->lesnu, st);
+  for (int j = 0; j < 3; ++j) {
+    hipLaunchKernelGGL(les_aux_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)s->lesnu,
+                       du[j], du[3 + j], du[6 + j], du[3 * j], du[3 * j + 1], du[3 * j + 2], T[9],
+                       T[10], T[11], N);
+    RC(der3(s, j, T[9], T[10], T[11], T[12], T[13], T[14], st));
+    hipLaunchKernelGGL(les_acc_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[12],
+                       (const double *)T[13], (const double *)T[14], s->lesf[0], s->lesf[1],
+                       s->lesf[2], (int)(j == 0), N);
+  }
+  NSCHK(hipGetLastError());
+  return MF_OK;
+}
+
 int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
   const int *it = s->it;
   const int n1 = it[0], L0 = it[NT - 1], L1 = it[NT - 2], L2 = it[NT - 3];
@@ -444,6 +520,9 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
     hipLaunchKernelGGL(flow_kernel, dim3(1), dim3(1024), 0, st, (const double *)s->part, s->nparts,
                        s->vol, fac1, s->fstate);
   }
+  // eddy viscosity and -div(tau) from the latest level (navier3d.f90:68)
+  if (s->les_type) RC(les_force(s, L0, st));
+  s->grad_level = -1;  // T is scratch from here on
   // boundary values of level it(1) with the tangential pressure gradient
   {
     BcArgs b;
@@ -468,6 +547,7 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
     r.c0 = cv[0]; r.c1 = cv[1]; r.c2 = cv[2];
     r.force = s->has_force[c] ? s->force[c] : nullptr;
     r.gflow = (c == 0 && s->flow) ? s->fstate : nullptr;
+    r.les = s->les_type ? s->lesf[c] : nullptr;
     r.re = s->re;
     r.out = s->T[0];
     r.N = N;
@@ -519,6 +599,7 @@ extern "C" int mf_ns_destroy(mf_ns *s) {
   mf_solver3d_destroy(s->scu);
   mf_solver3d_destroy(s->scp);
   if (s->mem) (void)hipFree(s->mem);
+  if (s->lesmem) (void)hipFree(s->lesmem);
   delete s;
   return MF_OK;
 }
@@ -532,6 +613,9 @@ extern "C" int mf_ns_create(const double *gx, int nx, const double *gy, int ny, 
   if (!(re > 0.0) || !(dt > 0.0)) return mf::fail(MF_ERR_ARG, "mf_ns_create: re and dt must be positive");
   mf_ns *s = new mf_ns();
   s->nx = nx; s->ny = ny; s->nz = nz;
+  s->gx.assign(gx, gx + nx);
+  s->gy.assign(gy, gy + ny);
+  s->gz.assign(gz, gz + nz);
   s->d = Dims{nx, ny, nz};
   s->N = (long)nx * ny * nz;
   s->nbc = 2L * ((long)(ny - 2) * (nz - 2) + (long)(nx - 2) * (nz - 2) + (long)(nx - 2) * (ny - 2));
@@ -645,6 +729,39 @@ extern "C" int mf_ns_set_flow_rate(mf_ns *s, int enable, double g0, void *stream
   return MF_OK;
 }
 
+extern "C" int mf_ns_set_les(mf_ns *s, int les_type, double c, void *stream) {
+  if (!s || les_type < 0 || les_type > 3 || !(c >= 0.0))
+    return mf::fail(MF_ERR_ARG, "mf_ns_set_les: les_type must be 0..3 and c >= 0");
+  if (les_type && !s->lesmem) {
+    const long Na = (s->N + 31) / 32 * 32;
+    const int ng = s->nx + s->ny + s->nz;
+    NSCHK(hipMalloc(&s->lesmem, (4 * Na + ng) * sizeof(double)));
+    double *m = (double *)s->lesmem;
+    for (int c3 = 0; c3 < 3; ++c3) s->lesf[c3] = m + c3 * Na;
+    s->lesnu = m + 3 * Na;
+    s->d2 = m + 4 * Na;
+    // squared spacings of the grids (navier_LES :1416-1446)
+    std::vector<double> h2(ng);
+    mf::les_spacing2(s->gx.data(), s->nx, h2.data());
+    mf::les_spacing2(s->gy.data(), s->ny, h2.data() + s->nx);
+    mf::les_spacing2(s->gz.data(), s->nz, h2.data() + s->nx + s->ny);
+    NSCHK(hipMemcpy(s->d2, h2.data(), ng * sizeof(double), hipMemcpyHostToDevice));
+    NSCHK(hipMemset(s->lesnu, 0, Na * sizeof(double)));
+  }
+  s->les_type = les_type;
+  s->les_c2 = c * c;  // navier_initialization: les_c = cmd%les_c**2 (:2138)
+  (void)stream;
+  return MF_OK;
+}
+
+extern "C" int mf_ns_les_nu(const mf_ns *s, double *nu, void *stream) {
+  if (!s || !nu) return mf::fail(MF_ERR_ARG, "mf_ns_les_nu: invalid argument");
+  if (!s->lesmem) return mf::fail(MF_ERR_ARG, "mf_ns_les_nu: LES was never enabled");
+  NSCHK(hipMemcpyAsync(nu, s->lesnu, s->N * sizeof(double), hipMemcpyDeviceToDevice,
+                       (hipStream_t)stream));
+  return MF_OK;
+}
+
 extern "C" int mf_ns_set_state(mf_ns *s, const double *u, const double *v, const double *w,
                                void *stream) {
   if (!s || !u || !v || !w) return mf::fail(MF_ERR_ARG, "mf_ns_set_state: null pointer");
@@ -664,6 +781,7 @@ extern "C" int mf_ns_set_state(mf_ns *s, const double *u, const double *v, const
   }
   for (int m = 0; m < NT; ++m) s->it[m] = m;
   s->time = 0.0;
+  s->grad_level = -1;
   NSCHK(hipGetLastError());
   return MF_OK;
 }

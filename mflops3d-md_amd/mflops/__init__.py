@@ -419,6 +419,16 @@ class NavierStokes:
         check(self._lib.mf_ns_info(self._h, C.byref(t), C.byref(g), _stream(stream)))
         return t.value, g.value
 
+    def set_les(self, les_type: int, c: float = 0.0, stream=None):
+        """Eddy-viscosity model (0 off, 1 Smagorinsky, 2 WALE, 3 sigma) with
+        constant c (squared inside, as navier_initialization does)."""
+        check(self._lib.mf_ns_set_les(self._h, int(les_type), float(c), _stream(stream)))
+
+    def les_nu(self, stream=None):
+        out = fortran_empty(self.shape)
+        check(self._lib.mf_ns_les_nu(self._h, C.c_void_p(_ptr(out)), _stream(stream)))
+        return out
+
 
 # ---------------------------------------------------------------------------
 # Multidomain (influence-matrix) solve

@@ -63,6 +63,8 @@ SIGNATURES = {
     "mf_ns_step": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _vp]),
     "mf_ns_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mf_ns_info": (C.c_int, [_vp, _dp, _dp, _vp]),
+    "mf_ns_set_les": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    "mf_ns_les_nu": (C.c_int, [_vp, _vp, _vp]),
     "mf_filter_create": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int,
                                    C.c_int, C.POINTER(_vp)]),
     "mf_filter_destroy": (C.c_int, [_vp]),

@@ -8,8 +8,10 @@ reference's defaults (command_line.f90:20-35): skew-symmetric nonlinear term
 (nlt=2), non-incremental rotational pressure correction with tangential
 boundary-gradient correction (pt=1), BDF order tou and pressure
 extrapolation order top-1 (tou, top in 1..3), no mapping (mapt=0), one
-sub-iteration, no LES (les_type=0: navier_LES computes velocity gradients
-whose values nothing reads, so it is omitted).  All derivatives are the
+sub-iteration.  LES (set_les) follows navier_LES (:1225-1452, via
+les_oracle.les_nu) and the LES part of f(nav,var) (:1083-1134): nu from the
+latest level's gradients du_ij = d u_i / d x_j (edges zeroed), then
+f_i += sum_j d_j (nu (du_ij + du_ji)).  All derivatives are the
 solver-mode compact derivatives (derivatives_coefficients_init(...,
 solver='yes'), class_navier_3D.f90:2287-2292) and all solves are solver_3d in
 FP64 (mf_oracle.solver_3d).
@@ -37,6 +39,7 @@ from __future__ import annotations
 
 import numpy as np
 
+import les_oracle as les
 import mf_oracle as o
 
 NT = 4
@@ -146,6 +149,25 @@ class NSOracle:
         self.W = integral_weights(*self.grids)
         self.it = list(range(NT))
         self.time = 0.0
+        self.les_type, self.les_c2, self.les_nu = 0, 0.0, None
+
+    def set_les(self, les_type: int, c: float):
+        """les_type 0..3 and the command-line constant (squared, :2138)."""
+        self.les_type, self.les_c2 = int(les_type), float(c) ** 2
+
+    def les_force(self, L: int):
+        """-les_f of the three components at level L (navier_LES + f)."""
+        q = [self.q[c][L] for c in "uvw"]
+        du = [[zero_edges(self.der(q[i], j)) for j in range(3)] for i in range(3)]
+        nu = les.les_nu(du, *self.grids, self.les_type, self.les_c2)
+        self.les_nu = nu
+        out = {}
+        for i, c in enumerate("uvw"):
+            acc = np.zeros(self.shape, order="F")
+            for j in range(3):
+                acc = acc + self.der(nu * (du[i][j] + du[j][i]), j)
+            out[c] = acc
+        return out
 
     # ---- set-up
     def set_state(self, u, v, w):
@@ -184,6 +206,8 @@ class NSOracle:
         self.time += self.dt
         # navier_bc_velocity: boundary values of level it(1)
         bc = {c: [b.copy() for b in self.bc[c]] for c in "uvw"}
+        # navier_LES on the latest level (navier3d.f90:68)
+        lesf = self.les_force(it[NT - 1]) if self.les_type else None
         # presolve: rhs_c = f_c - extrapol_v(N_c)
         rhs = {}
         for c in "uvw":
@@ -198,6 +222,8 @@ class NSOracle:
                 ue = extrapol(self.q["u"], it, self.tou)
                 qf = float((self.W * ue).sum() / self.W.sum())
                 f = f + self.flow.update(qf, fac[0])
+            if lesf is not None:
+                f = f + lesf[c]
             rhs[c] = zero_edges(f - extrapol(lv, it, self.tou))
         # add_boundary_gradient (pt=1): tangential bc += d(extrapol_p(phi))/fac1
         pe = extrapol(self.phi, it, self.top - 1)
