@@ -65,7 +65,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-solves", type=int, default=2)
-    ap.add_argument("--workload", choices=("helmholtz", "influence", "md"), default="helmholtz")
+    ap.add_argument("--workload", choices=("helmholtz", "influence", "md", "sgs"),
+                    default="helmholtz")
+    ap.add_argument("--sgs-fine", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--sgs-coarse", type=int, nargs=3, default=(64, 64, 64))
+    ap.add_argument("--sgs-steps", type=int, default=200)
+    ap.add_argument("--sgs-snap", type=int, default=50)
+    ap.add_argument("--sgs-train", type=int, default=300)
     ap.add_argument("--ns-steps", type=int, default=10, help="timed NS steps (0: skip)")
     ap.add_argument("--ns-warmup", type=int, default=3)
     ap.add_argument("--ns-shape", type=int, nargs=3, default=(192, 256, 192))
@@ -81,6 +87,8 @@ def main():
         return run_influence(args)
     if args.workload == "md":
         return run_md(args)
+    if args.workload == "sgs":
+        return run_sgs(args)
 
     import numpy as np
     import torch
@@ -309,6 +317,47 @@ def cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx):
     base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": cores, "kind": "port",
             "sample": "%d full %d^3 FP64 solves (numpy/OpenBLAS dgemm)" % (args.cpu_solves, n)}
     return base, errs
+
+
+def run_sgs(args):
+    """Config 5, the learned SGS closure: a fine channel run (default 256^3,
+    Re 5000, tanh stretch 2.5 in y, perturbation 0.3), snapshots filtered by
+    the LSQ spline onto the coarse grid (default 64^3; nodes = output points)
+    give the training set (features: filtered gradients and width; targets:
+    tau_ij), the MLP 10-64-64-6 is trained with full-batch Adam, and coarse
+    runs with no model, Smagorinsky (C_s 0.165) and the learned closure start
+    from the filtered initial field.  value = the learned closure's
+    a-posteriori error (relative L2 of the velocity against the filtered fine
+    run, mean over u, v, w) after the same number of steps."""
+    import torch
+
+    import mflops as mf
+    from mflops import sgs
+
+    torch.cuda.set_device(0)
+    res = sgs.run(mf, fine_shape=tuple(args.sgs_fine), coarse_shape=tuple(args.sgs_coarse),
+                  re=5000.0, steps=args.sgs_steps, snap_every=args.sgs_snap,
+                  train_steps=args.sgs_train, log=lambda m: log("sgs: " + m))
+    mean = {k: sum(v) / 3.0 for k, v in res.errors.items()}
+    out = {
+        "metric": "learned SGS a-posteriori error vs the filtered fine run (config 5)",
+        "value": round(mean["learned"], 6),
+        "unit": "relative L2 error",
+        "n_gpus": 1,
+        "steps": args.sgs_steps,
+        "higher_is_better": False,
+        "dtype": "f64 (closure f32, bf16 weights)",
+        "data": "synthetic",
+        "config": {"workload": "sgs-channel-%dx%dx%d-to-%dx%dx%d-re5000"
+                   % (tuple(args.sgs_fine) + tuple(args.sgs_coarse)),
+                   "snapshot_every": args.sgs_snap, "adam_steps": args.sgs_train,
+                   "samples": res.samples, "mlp": "10-64-64-6 tanh"},
+        "errors": {k: [round(e, 6) for e in v] for k, v in res.errors.items()},
+        "error_mean": {k: round(v, 6) for k, v in mean.items()},
+        "training_loss": [round(res.loss[0], 5), round(res.loss[1], 5)],
+        "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_md(args):

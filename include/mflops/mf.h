@@ -344,6 +344,33 @@ int mf_mlp_loss_grad(mf_mlp *h, const float *X, const float *T, long long n, flo
 int mf_mlp_adam_step(mf_mlp *h, const float *X, const float *T, long long n, float lr,
                      float *loss, void *stream);
 
+/* Training data of the closure from a fine field (the use the reference's
+ * filter module serves, class_filter.f90): ubar = F u, vbar, wbar on the
+ * filter's output grid and tau (n x 6 float, row-major, n the output
+ * points in Fortran order) with tau_ij = (F(u_i u_j) - F(u_i) F(u_j)) /
+ * tscale, columns xx, yy, zz, xy, xz, yz.  work holds nin_x nin_y nin_z +
+ * nout_x nout_y nout_z + nout_x nin_y nin_z + nout_x nout_y nin_z doubles. */
+int mf_sgs_targets(const mf_filter *fx, const mf_filter *fy, const mf_filter *fz, const double *u,
+                   const double *v, const double *w, double *ubar, double *vbar, double *wbar,
+                   float *tau, double tscale, double *work, void *stream);
+/* Closure inputs X (n x 10 float) of a velocity field (nx,ny,nz): the
+ * gradients du_i/dx_j (row-major ij, derivative handles in solver mode, as
+ * navier_LES takes them, zero on the box edges) times gscale, and
+ * lscale * Delta with Delta^2 = (dx^2+dy^2+dz^2)/3 (host grids).  work holds
+ * 9 nx ny nz doubles.  Synchronises the stream. */
+int mf_sgs_features(const mf_der *dx, const mf_der *dy, const mf_der *dz, const double *u,
+                    const double *v, const double *w, int nx, int ny, int nz, const double *gx,
+                    const double *gy, const double *gz, double gscale, double lscale, float *X,
+                    double *work, void *stream);
+/* The learned closure in the NS step (tau enters the momentum equation as
+ * -div tau, as the eddy-viscosity stress does in f(nav,var) :1083-1134):
+ * each step takes X of the latest level as mf_sgs_features does, tau =
+ * tscale * mlp(X), and adds -sum_j d_j tau_ij (tau zero on the box edges)
+ * to the force of u_i, on top of any eddy viscosity.  mlp NULL turns it
+ * off; the handle must outlive its use. */
+int mf_ns_set_sgs(mf_ns *ns, const mf_mlp *mlp, double gscale, double lscale, double tscale,
+                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

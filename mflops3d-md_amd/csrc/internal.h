@@ -55,6 +55,15 @@ void les_nu_launch(const double *const du[9], const double *d2x, const double *d
                    const double *d2z, int nx, int ny, int nz, int type, double c, double *nu,
                    void *stream);
 
+// Learned-closure kernels (les.hip): features X (n x 10 float) from the
+// gradients, and o_i = -tscale tau_ij (edges zero) of direction j from the
+// closure's outputs Y (n x 6 float: xx, yy, zz, xy, xz, yz).
+void sgs_features_launch(const double *const du[9], const double *d2x, const double *d2y,
+                         const double *d2z, int nx, int ny, int nz, double gscale, double lscale,
+                         float *X, void *stream);
+void sgs_aux_launch(const float *Y, int j, double tscale, double *o0, double *o1, double *o2,
+                    int nx, int ny, int nz, void *stream);
+
 // thread-local last error message (mf_last_error)
 int fail(int code, const std::string &msg);
 

@@ -311,8 +311,87 @@ __global__ void __launch_bounds__(256) les_nu_kernel(NuArgs a) {
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
+// ---- learned-closure data path ----
+__global__ void __launch_bounds__(256) prod_kernel(const double *__restrict__ a,
+                                                   const double *__restrict__ b,
+                                                   double *__restrict__ o, long N) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < N) o[t] = a[t] * b[t];
+}
+
+// T[t][col] = (F(u_i u_j) - F(u_i) F(u_j)) / tscale
+__global__ void __launch_bounds__(256) tau_kernel(const double *__restrict__ uu,
+                                                  const double *__restrict__ ui,
+                                                  const double *__restrict__ uj,
+                                                  float *__restrict__ T, int col, double inv, long N) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < N) T[t * O + col] = (float)((uu[t] - ui[t] * uj[t]) * inv);
+}
+
+struct FeatArgs {
+  const double *du[9];  // du[3 i + j] = d u_i / d x_j
+  const double *d2x, *d2y, *d2z;
+  int nx, ny, nz;
+  double gscale, lscale;
+  float *X;
+};
+
+// X[t] = (gscale du_ij (row-major ij), lscale sqrt((dx^2+dy^2+dz^2)/3));
+// gradients zero on the box edges (field_zero_edges)
+__global__ void __launch_bounds__(256) features_kernel(FeatArgs a) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long N = (long)a.nx * a.ny * a.nz;
+  if (t >= N) return;
+  const int i = (int)(t % a.nx), j = (int)((t / a.nx) % a.ny), k = (int)(t / ((long)a.nx * a.ny));
+  const int nb = (i == 0 || i == a.nx - 1) + (j == 0 || j == a.ny - 1) + (k == 0 || k == a.nz - 1);
+  float *x = a.X + t * I0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) x[q] = nb >= 2 ? 0.0f : (float)(a.gscale * a.du[q][t]);
+  x[9] = (float)(a.lscale * sqrt((a.d2x[i] + a.d2y[j] + a.d2z[k]) / 3.0));
+}
+
+// column of tau_ij in the six outputs (xx, yy, zz, xy, xz, yz)
+__device__ __forceinline__ int pair_col(int i, int j) {
+  return i == j ? i : (i + j == 1 ? 3 : (i + j == 2 ? 4 : 5));
+}
+
+// o_i = -tscale Y[t][pair(i, j)] for one direction j, zero on the box edges
+__global__ void __launch_bounds__(256) sgs_aux_kernel(const float *__restrict__ Y, int jd,
+                                                      double tscale, double *__restrict__ o0,
+                                                      double *__restrict__ o1,
+                                                      double *__restrict__ o2, int nx, int ny,
+                                                      int nz) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long N = (long)nx * ny * nz;
+  if (t >= N) return;
+  const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((long)nx * ny));
+  const int nb = (i == 0 || i == nx - 1) + (j == 0 || j == ny - 1) + (k == 0 || k == nz - 1);
+  const float *y = Y + t * O;
+  const double s = nb >= 2 ? 0.0 : -tscale;
+  o0[t] = s * (double)y[pair_col(0, jd)];
+  o1[t] = s * (double)y[pair_col(1, jd)];
+  o2[t] = s * (double)y[pair_col(2, jd)];
+}
+
 }  // namespace les
 }  // namespace mf
+
+void mf::sgs_features_launch(const double *const du[9], const double *d2x, const double *d2y,
+                             const double *d2z, int nx, int ny, int nz, double gscale,
+                             double lscale, float *X, void *stream) {
+  mf::les::FeatArgs a;
+  for (int q = 0; q < 9; ++q) a.du[q] = du[q];
+  a.d2x = d2x; a.d2y = d2y; a.d2z = d2z;
+  a.nx = nx; a.ny = ny; a.nz = nz; a.gscale = gscale; a.lscale = lscale; a.X = X;
+  hipLaunchKernelGGL(mf::les::features_kernel, dim3(mf::les::nblk((long)nx * ny * nz)), dim3(256),
+                     0, (hipStream_t)stream, a);
+}
+
+void mf::sgs_aux_launch(const float *Y, int j, double tscale, double *o0, double *o1, double *o2,
+                        int nx, int ny, int nz, void *stream) {
+  hipLaunchKernelGGL(mf::les::sgs_aux_kernel, dim3(mf::les::nblk((long)nx * ny * nz)), dim3(256),
+                     0, (hipStream_t)stream, Y, j, tscale, o0, o1, o2, nx, ny, nz);
+}
 
 // ---------------------------------------------------------------------------
 // Filter handle
@@ -688,5 +767,75 @@ extern "C" int mf_mlp_adam_step(mf_mlp *h, const float *X, const float *T, long 
     LESCHK(hipMemcpyAsync(loss, h->g + NPARAM, sizeof(float), hipMemcpyDeviceToHost, st));
     LESCHK(hipStreamSynchronize(st));
   }
+  return MF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned-closure data path
+// ---------------------------------------------------------------------------
+extern "C" int mf_sgs_targets(const mf_filter *fx, const mf_filter *fy, const mf_filter *fz,
+                              const double *u, const double *v, const double *w, double *ubar,
+                              double *vbar, double *wbar, float *tau, double tscale, double *work,
+                              void *stream) {
+  using namespace mf::les;
+  if (!fx || !fy || !fz || !u || !v || !w || !ubar || !vbar || !wbar || !tau || !work)
+    return mf::fail(MF_ERR_ARG, "mf_sgs_targets: null pointer");
+  if (!(tscale > 0.0)) return mf::fail(MF_ERR_ARG, "mf_sgs_targets: tscale must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  const long Nf = (long)fx->nin * fy->nin * fz->nin;
+  const long Nc = (long)fx->nout * fy->nout * fz->nout;
+  double *prod = work, *fb = work + Nf;
+  double *fw = fb + Nc;  // filter work
+  const double *q[3] = {u, v, w};
+  double *qb[3] = {ubar, vbar, wbar};
+  for (int c = 0; c < 3; ++c) {
+    int rc = mf_filter3d(fx, fy, fz, q[c], qb[c], fw, st);
+    if (rc != MF_OK) return rc;
+  }
+  const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
+  for (int col = 0; col < 6; ++col) {
+    hipLaunchKernelGGL(prod_kernel, dim3(nblk(Nf)), dim3(256), 0, st, q[pi[col]], q[pj[col]], prod, Nf);
+    int rc = mf_filter3d(fx, fy, fz, prod, fb, fw, st);
+    if (rc != MF_OK) return rc;
+    hipLaunchKernelGGL(tau_kernel, dim3(nblk(Nc)), dim3(256), 0, st, (const double *)fb,
+                       (const double *)qb[pi[col]], (const double *)qb[pj[col]], tau, col,
+                       1.0 / tscale, Nc);
+  }
+  LESCHK(hipGetLastError());
+  return MF_OK;
+}
+
+extern "C" int mf_sgs_features(const mf_der *dx, const mf_der *dy, const mf_der *dz,
+                               const double *u, const double *v, const double *w, int nx, int ny,
+                               int nz, const double *gx, const double *gy, const double *gz,
+                               double gscale, double lscale, float *X, double *work,
+                               void *stream) {
+  if (!dx || !dy || !dz || !u || !v || !w || !gx || !gy || !gz || !X || !work)
+    return mf::fail(MF_ERR_ARG, "mf_sgs_features: null pointer");
+  if (nx < 2 || ny < 2 || nz < 2) return mf::fail(MF_ERR_ARG, "mf_sgs_features: sizes must be >= 2");
+  hipStream_t st = (hipStream_t)stream;
+  const long N = (long)nx * ny * nz;
+  double *g[9];  // g[3 a + c] = d q_c / d x_a
+  for (int q = 0; q < 9; ++q) g[q] = work + q * N;
+  const double *in[3] = {u, v, w};
+  const mf_der *dc[3] = {dx, dy, dz};
+  for (int a = 0; a < 3; ++a) {
+    int rc = mf_der3d_batch(dc[a], a, 1, 3, in, g + 3 * a, nx, ny, nz, st);
+    if (rc != MF_OK) return rc;
+  }
+  const double *du[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) du[3 * i + j] = g[3 * j + i];
+  std::vector<double> d2((size_t)nx + ny + nz);
+  mf::les_spacing2(gx, nx, d2.data());
+  mf::les_spacing2(gy, ny, d2.data() + nx);
+  mf::les_spacing2(gz, nz, d2.data() + nx + ny);
+  double *dd = nullptr;
+  LESCHK(hipMallocAsync(&dd, d2.size() * sizeof(double), st));
+  LESCHK(hipMemcpyAsync(dd, d2.data(), d2.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  mf::sgs_features_launch(du, dd, dd + nx, dd + nx + ny, nx, ny, nz, gscale, lscale, X, st);
+  LESCHK(hipGetLastError());
+  LESCHK(hipFreeAsync(dd, st));
+  LESCHK(hipStreamSynchronize(st));  // d2 goes out of scope
   return MF_OK;
 }

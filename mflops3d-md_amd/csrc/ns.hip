@@ -401,6 +401,11 @@ struct mf_ns {
   double les_c2 = 0.0;
   void *lesmem = nullptr;
   double *lesf[3] = {nullptr, nullptr, nullptr}, *lesnu = nullptr, *d2 = nullptr;
+  // learned closure: tau = tscale MLP(gscale du, lscale Delta)
+  const mf_mlp *sgs = nullptr;
+  double sgs_g = 1.0, sgs_l = 1.0, sgs_t = 1.0;
+  void *sgsmem = nullptr;
+  float *sgsX = nullptr, *sgsY = nullptr;
 };
 
 namespace {
@@ -484,16 +489,33 @@ int les_force(mf_ns *s, int L, hipStream_t st) {
   const double *du[9];  // du[3 i + j] = d u_i / d x_j = T[3 j + i]
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) du[3 * i + j] = T[3 * j + i];
-  mf::les_nu_launch(du, s->d2, s->d2 + s->nx, s->d2 + s->nx + s->ny, s->nx, s->ny, s->nz,
-                    s->les_type, s->les_c2, s->lesnu, st);
-  for (int j = 0; j < 3; ++j) {
-    hipLaunchKernelGGL(les_aux_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)s->lesnu,
-                       du[j], du[3 + j], du[6 + j], du[3 * j], du[3 * j + 1], du[3 * j + 2], T[9],
-                       T[10], T[11], N);
-    RC(der3(s, j, T[9], T[10], T[11], T[12], T[13], T[14], st));
-    hipLaunchKernelGGL(les_acc_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[12],
-                       (const double *)T[13], (const double *)T[14], s->lesf[0], s->lesf[1],
-                       s->lesf[2], (int)(j == 0), N);
+  bool first = true;
+  if (s->les_type) {
+    mf::les_nu_launch(du, s->d2, s->d2 + s->nx, s->d2 + s->nx + s->ny, s->nx, s->ny, s->nz,
+                      s->les_type, s->les_c2, s->lesnu, st);
+    for (int j = 0; j < 3; ++j) {
+      hipLaunchKernelGGL(les_aux_kernel, dim3(nblk(N)), dim3(256), 0, st,
+                         (const double *)s->lesnu, du[j], du[3 + j], du[6 + j], du[3 * j],
+                         du[3 * j + 1], du[3 * j + 2], T[9], T[10], T[11], N);
+      RC(der3(s, j, T[9], T[10], T[11], T[12], T[13], T[14], st));
+      hipLaunchKernelGGL(les_acc_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[12],
+                         (const double *)T[13], (const double *)T[14], s->lesf[0], s->lesf[1],
+                         s->lesf[2], (int)first, N);
+      first = false;
+    }
+  }
+  if (s->sgs) {  // -div tau of the learned closure
+    mf::sgs_features_launch(du, s->d2, s->d2 + s->nx, s->d2 + s->nx + s->ny, s->nx, s->ny, s->nz,
+                            s->sgs_g, s->sgs_l, s->sgsX, st);
+    RC(mf_mlp_forward(s->sgs, s->sgsX, s->sgsY, N, st));
+    for (int j = 0; j < 3; ++j) {
+      mf::sgs_aux_launch(s->sgsY, j, s->sgs_t, T[9], T[10], T[11], s->nx, s->ny, s->nz, st);
+      RC(der3(s, j, T[9], T[10], T[11], T[12], T[13], T[14], st));
+      hipLaunchKernelGGL(les_acc_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[12],
+                         (const double *)T[13], (const double *)T[14], s->lesf[0], s->lesf[1],
+                         s->lesf[2], (int)first, N);
+      first = false;
+    }
   }
   NSCHK(hipGetLastError());
   return MF_OK;
@@ -521,7 +543,7 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
                        s->vol, fac1, s->fstate);
   }
   // eddy viscosity and -div(tau) from the latest level (navier3d.f90:68)
-  if (s->les_type) RC(les_force(s, L0, st));
+  if (s->les_type || s->sgs) RC(les_force(s, L0, st));
   s->grad_level = -1;  // T is scratch from here on
   // boundary values of level it(1) with the tangential pressure gradient
   {
@@ -547,7 +569,7 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
     r.c0 = cv[0]; r.c1 = cv[1]; r.c2 = cv[2];
     r.force = s->has_force[c] ? s->force[c] : nullptr;
     r.gflow = (c == 0 && s->flow) ? s->fstate : nullptr;
-    r.les = s->les_type ? s->lesf[c] : nullptr;
+    r.les = (s->les_type || s->sgs) ? s->lesf[c] : nullptr;
     r.re = s->re;
     r.out = s->T[0];
     r.N = N;
@@ -600,6 +622,7 @@ extern "C" int mf_ns_destroy(mf_ns *s) {
   mf_solver3d_destroy(s->scp);
   if (s->mem) (void)hipFree(s->mem);
   if (s->lesmem) (void)hipFree(s->lesmem);
+  if (s->sgsmem) (void)hipFree(s->sgsmem);
   delete s;
   return MF_OK;
 }
@@ -729,10 +752,10 @@ extern "C" int mf_ns_set_flow_rate(mf_ns *s, int enable, double g0, void *stream
   return MF_OK;
 }
 
-extern "C" int mf_ns_set_les(mf_ns *s, int les_type, double c, void *stream) {
-  if (!s || les_type < 0 || les_type > 3 || !(c >= 0.0))
-    return mf::fail(MF_ERR_ARG, "mf_ns_set_les: les_type must be 0..3 and c >= 0");
-  if (les_type && !s->lesmem) {
+namespace {
+// force accumulators, nu and the squared spacings (first use of LES or SGS)
+int les_alloc(mf_ns *s) {
+  if (!s->lesmem) {
     const long Na = (s->N + 31) / 32 * 32;
     const int ng = s->nx + s->ny + s->nz;
     NSCHK(hipMalloc(&s->lesmem, (4 * Na + ng) * sizeof(double)));
@@ -748,8 +771,36 @@ extern "C" int mf_ns_set_les(mf_ns *s, int les_type, double c, void *stream) {
     NSCHK(hipMemcpy(s->d2, h2.data(), ng * sizeof(double), hipMemcpyHostToDevice));
     NSCHK(hipMemset(s->lesnu, 0, Na * sizeof(double)));
   }
+  return MF_OK;
+}
+}  // namespace
+
+extern "C" int mf_ns_set_les(mf_ns *s, int les_type, double c, void *stream) {
+  if (!s || les_type < 0 || les_type > 3 || !(c >= 0.0))
+    return mf::fail(MF_ERR_ARG, "mf_ns_set_les: les_type must be 0..3 and c >= 0");
+  if (les_type) RC(les_alloc(s));
   s->les_type = les_type;
   s->les_c2 = c * c;  // navier_initialization: les_c = cmd%les_c**2 (:2138)
+  (void)stream;
+  return MF_OK;
+}
+
+extern "C" int mf_ns_set_sgs(mf_ns *s, const mf_mlp *mlp, double gscale, double lscale,
+                             double tscale, void *stream) {
+  if (!s) return mf::fail(MF_ERR_ARG, "mf_ns_set_sgs: null handle");
+  if (mlp && !(tscale > 0.0)) return mf::fail(MF_ERR_ARG, "mf_ns_set_sgs: tscale must be positive");
+  if (mlp) {
+    RC(les_alloc(s));
+    if (!s->sgsmem) {
+      NSCHK(hipMalloc(&s->sgsmem, (size_t)s->N * 16 * sizeof(float)));
+      s->sgsX = (float *)s->sgsmem;
+      s->sgsY = s->sgsX + (size_t)s->N * 10;
+    }
+  }
+  s->sgs = mlp;
+  s->sgs_g = gscale;
+  s->sgs_l = lscale;
+  s->sgs_t = tscale;
   (void)stream;
   return MF_OK;
 }

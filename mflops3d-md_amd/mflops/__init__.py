@@ -424,6 +424,15 @@ class NavierStokes:
         constant c (squared inside, as navier_initialization does)."""
         check(self._lib.mf_ns_set_les(self._h, int(les_type), float(c), _stream(stream)))
 
+    def set_sgs(self, net, gscale: float = 1.0, lscale: float = 1.0, tscale: float = 1.0,
+                stream=None):
+        """Learned closure (an SGSNet, or None to turn it off): tau =
+        tscale * net(gscale du, lscale Delta), added as -div tau."""
+        self._sgs = net  # keep the network alive while the step uses it
+        h = net._h if net is not None else C.c_void_p()
+        check(self._lib.mf_ns_set_sgs(self._h, h, float(gscale), float(lscale), float(tscale),
+                                      _stream(stream)))
+
     def les_nu(self, stream=None):
         out = fortran_empty(self.shape)
         check(self._lib.mf_ns_les_nu(self._h, C.c_void_p(_ptr(out)), _stream(stream)))
@@ -647,3 +656,43 @@ class SGSNet:
         check(self._lib.mf_mlp_adam_step(self._h, self._f32(X, 10), self._f32(T, 6), X.shape[0],
                                          float(lr), C.byref(loss), _stream(stream)))
         return loss.value
+
+
+def sgs_targets(fx: SplineFilter, fy: SplineFilter, fz: SplineFilter, u, v, w,
+                tscale: float = 1.0, stream=None):
+    """(ubar, vbar, wbar), tau (n x 6 float32 / tscale) of a fine field."""
+    torch = _torch()
+    shape_o = (fx.nout, fy.nout, fz.nout)
+    nf = fx.nin * fy.nin * fz.nin
+    nc = fx.nout * fy.nout * fz.nout
+    bars = [fortran_empty(shape_o, u.device) for _ in range(3)]
+    tau = torch.empty((nc, 6), dtype=torch.float32, device=u.device)
+    work = torch.empty(nf + nc + fx.nout * fy.nin * fz.nin + fx.nout * fy.nout * fz.nin,
+                       dtype=torch.float64, device=u.device)
+    for t in (u, v, w):
+        if tuple(t.shape) != (fx.nin, fy.nin, fz.nin):
+            raise ValueError("field shape %s != filter input" % (tuple(t.shape),))
+    check(fx._lib.mf_sgs_targets(fx._h, fy._h, fz._h, *[C.c_void_p(_ptr(t)) for t in (u, v, w)],
+                                 *[C.c_void_p(_ptr(t)) for t in bars], C.c_void_p(tau.data_ptr()),
+                                 float(tscale), C.c_void_p(work.data_ptr()), _stream(stream)))
+    return bars, tau
+
+
+def sgs_features(dcs, u, v, w, grids, gscale: float = 1.0, lscale: float = 1.0, stream=None):
+    """Closure inputs (n x 10 float32) of a velocity field; dcs are solver-mode
+    DerivativeCoefficients of the three grids."""
+    torch = _torch()
+    lib = load()
+    gs = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
+    shape = tuple(len(g) for g in gs)
+    for t in (u, v, w):
+        if tuple(t.shape) != shape:
+            raise ValueError("field shape %s != %s" % (tuple(t.shape), shape))
+    n = shape[0] * shape[1] * shape[2]
+    X = torch.empty((n, 10), dtype=torch.float32, device=u.device)
+    work = torch.empty(9 * n, dtype=torch.float64, device=u.device)
+    check(lib.mf_sgs_features(*[d._h for d in dcs], *[C.c_void_p(_ptr(t)) for t in (u, v, w)],
+                              *shape, *[_dptr(g) for g in gs], float(gscale), float(lscale),
+                              C.c_void_p(X.data_ptr()), C.c_void_p(work.data_ptr()),
+                              _stream(stream)))
+    return X

@@ -78,6 +78,11 @@ SIGNATURES = {
     "mf_mlp_forward": (C.c_int, [_vp, _vp, _vp, C.c_longlong, _vp]),
     "mf_mlp_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp]),
     "mf_mlp_adam_step": (C.c_int, [_vp, _vp, _vp, C.c_longlong, C.c_float, _vp, _vp]),
+    "mf_sgs_targets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                 _vp, _vp]),
+    "mf_sgs_features": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                  _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp]),
+    "mf_ns_set_sgs": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, _vp]),
 }
 
 PROF_CLASSES = ("fold", "transform_fp64", "transform_fp32", "residual_fp64",

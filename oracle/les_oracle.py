@@ -18,6 +18,12 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ (and nothing in the product).
   box edges.  Pinned by closed forms (Smagorinsky on simple shear) and by the
   models' defining properties (WALE vanishes for pure shear, sigma for
   two-dimensional gradients and solid rotation).
+* sgs_targets / sgs_features: the closure's training data from a fine field
+  (the filter module's purpose, class_filter.f90): filtered velocity and
+  tau_ij = F(u_i u_j) - F(u_i) F(u_j); inputs are the gradients du_ij (edges
+  zeroed, as navier_LES zeroes them) and the filter width.  Pinned by the
+  filter's polynomial reproduction (tau = 0 for a linear field when k >= 2,
+  and for any field when the filter is the identity).
 * MLP: the learned closure 10 -> 64 -> 64 -> 6 (tanh hidden layers) on
   features (9 velocity gradients, filter width) to the 6 SGS stresses, with
   bf16-rounded weights as the device kernels use them.  Its gradients are
@@ -171,6 +177,40 @@ def les_nu(du, gx, gy, gz, les_type: int, c: float):
     dx, dy, dz = (_spacing(g) ** 2 for g in (gx, gy, gz))
     nu = nu * (dx[:, None, None] + dy[None, :, None] + dz[None, None, :]) / 3.0
     return zero_edges(nu)
+
+
+# ----------------------------------------------------------------------------
+# Closure data
+# ----------------------------------------------------------------------------
+
+PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))  # tau columns
+
+
+def pair_col(i: int, j: int) -> int:
+    return PAIRS.index((min(i, j), max(i, j)))
+
+
+def filter3d(Fx, Fy, Fz, f):
+    return np.einsum("ai,bj,ck,ijk->abc", Fx, Fy, Fz, f, optimize=True)
+
+
+def sgs_targets(Fx, Fy, Fz, u, v, w, tscale: float = 1.0):
+    """(ubar, vbar, wbar) and tau (n x 6, rows in Fortran point order) / tscale."""
+    q = [np.asarray(a, dtype=np.float64) for a in (u, v, w)]
+    qb = [filter3d(Fx, Fy, Fz, a) for a in q]
+    tau = np.stack([(filter3d(Fx, Fy, Fz, q[i] * q[j]) - qb[i] * qb[j]).ravel(order="F")
+                    for i, j in PAIRS], 1) / tscale
+    return qb, tau
+
+
+def sgs_features(du, gx, gy, gz, gscale: float = 1.0, lscale: float = 1.0):
+    """X (n x 10): gscale du_ij (row-major ij, edges zero), lscale Delta."""
+    cols = [gscale * zero_edges(np.asarray(du[i][j], dtype=np.float64)).ravel(order="F")
+            for i in range(3) for j in range(3)]
+    dx, dy, dz = (_spacing(g) ** 2 for g in (gx, gy, gz))
+    d = np.sqrt((dx[:, None, None] + dy[None, :, None] + dz[None, None, :]) / 3.0)
+    cols.append(lscale * d.ravel(order="F"))
+    return np.stack(cols, 1)
 
 
 # ----------------------------------------------------------------------------

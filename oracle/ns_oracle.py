@@ -150,23 +150,37 @@ class NSOracle:
         self.it = list(range(NT))
         self.time = 0.0
         self.les_type, self.les_c2, self.les_nu = 0, 0.0, None
+        self.sgs = None
+
+    def set_sgs(self, params, gscale: float, lscale: float, tscale: float):
+        """Learned closure: tau = tscale MLP(X), X as les.sgs_features (rounded
+        to float32, as the device stores it), added as -div tau."""
+        self.sgs = (np.asarray(params, dtype=np.float32), gscale, lscale, tscale)
 
     def set_les(self, les_type: int, c: float):
         """les_type 0..3 and the command-line constant (squared, :2138)."""
         self.les_type, self.les_c2 = int(les_type), float(c) ** 2
 
     def les_force(self, L: int):
-        """-les_f of the three components at level L (navier_LES + f)."""
+        """-les_f of the three components at level L (navier_LES + f), plus
+        -div tau of the learned closure when set."""
         q = [self.q[c][L] for c in "uvw"]
         du = [[zero_edges(self.der(q[i], j)) for j in range(3)] for i in range(3)]
-        nu = les.les_nu(du, *self.grids, self.les_type, self.les_c2)
-        self.les_nu = nu
-        out = {}
-        for i, c in enumerate("uvw"):
-            acc = np.zeros(self.shape, order="F")
-            for j in range(3):
-                acc = acc + self.der(nu * (du[i][j] + du[j][i]), j)
-            out[c] = acc
+        out = {c: np.zeros(self.shape, order="F") for c in "uvw"}
+        if self.les_type:
+            nu = les.les_nu(du, *self.grids, self.les_type, self.les_c2)
+            self.les_nu = nu
+            for i, c in enumerate("uvw"):
+                for j in range(3):
+                    out[c] = out[c] + self.der(nu * (du[i][j] + du[j][i]), j)
+        if self.sgs is not None:
+            p, gs, ls, ts = self.sgs
+            X = les.sgs_features(du, *self.grids, gs, ls).astype(np.float32)
+            Y = les.forward(p, X)[0]
+            for i, c in enumerate("uvw"):
+                for j in range(3):
+                    tau = ts * Y[:, les.pair_col(i, j)].reshape(self.shape, order="F")
+                    out[c] = out[c] - self.der(zero_edges(tau), j)
         return out
 
     # ---- set-up
@@ -207,7 +221,7 @@ class NSOracle:
         # navier_bc_velocity: boundary values of level it(1)
         bc = {c: [b.copy() for b in self.bc[c]] for c in "uvw"}
         # navier_LES on the latest level (navier3d.f90:68)
-        lesf = self.les_force(it[NT - 1]) if self.les_type else None
+        lesf = self.les_force(it[NT - 1]) if (self.les_type or self.sgs is not None) else None
         # presolve: rhs_c = f_c - extrapol_v(N_c)
         rhs = {}
         for c in "uvw":

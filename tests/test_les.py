@@ -135,6 +135,52 @@ def test_oracle_models_are_rotation_invariant(les_type):
         assert abs(a - b) < 1e-10 * abs(a)
 
 
+# ----------------------------------------------------------- closure data ---
+
+def _filters(shape_in, shape_out, nodes, k=3):
+    Fs, axes = [], []
+    for a in range(3):
+        xi = _grid_stretched(shape_in[a], 1.5)
+        xo = np.linspace(0, 1, shape_out[a])
+        xn = np.linspace(0, 1, nodes[a])
+        axes.append((xi, xo, xn))
+        Fs.append(L.lsq_filter_matrix(xi, xo, xn, k))
+    return Fs, axes
+
+
+def test_oracle_sgs_targets_vanish_for_linear_fields():
+    """Products of linear fields are quadratic, which a cubic spline fit
+    reproduces: tau = 0 and ubar is the field at the output points."""
+    Fs, axes = _filters((30, 26, 22), (9, 8, 7), (7, 6, 5))
+    X, Y, Z = np.meshgrid(*[a[0] for a in axes], indexing="ij")
+    Xo, Yo, Zo = np.meshgrid(*[a[1] for a in axes], indexing="ij")
+    lin = lambda X, Y, Z: [1 + 2 * X - Y, 0.5 * Y + 3 * Z, X - Z + 0.25]
+    qb, tau = L.sgs_targets(*Fs, *lin(X, Y, Z))
+    for a, b in zip(qb, lin(Xo, Yo, Zo)):
+        assert np.abs(a - b).max() < 1e-12
+    assert np.abs(tau).max() < 1e-11
+    # an identity filter (linear spline with a knot at every point, output on
+    # the input points) leaves no stress for any field
+    rng = np.random.default_rng(0)
+    xi = [_grid_stretched(n) for n in (9, 8, 7)]
+    Fi = [L.lsq_filter_matrix(g, g, g, 1) for g in xi]
+    assert max(np.abs(F - np.eye(len(F))).max() for F in Fi) < 1e-13
+    _, tau = L.sgs_targets(*Fi, *[rng.standard_normal((9, 8, 7)) for _ in range(3)])
+    assert np.abs(tau).max() < 1e-12
+
+
+def test_oracle_sgs_features_layout():
+    gx, gy, gz = _grid_stretched(7), np.linspace(0, 2, 6), _grid_stretched(5, 1.0)
+    A = np.array([[0.3, -1.0, 2.0], [0.5, 0.1, -0.7], [1.5, 0.2, -0.4]])
+    du = _const_du(A, (7, 6, 5))
+    Xf = L.sgs_features(du, gx, gy, gz, gscale=2.0, lscale=3.0)
+    d = np.sqrt(_delta2(gx, gy, gz))
+    inner = _interior(np.arange(7 * 6 * 5).reshape((7, 6, 5), order="F")).ravel()
+    assert np.array_equal(Xf[inner, :9], np.tile(2.0 * A.ravel(), (len(inner), 1)))
+    assert np.abs(Xf[:, 9] - 3.0 * d.ravel(order="F")).max() < 1e-15
+    assert np.abs(Xf[0, :9]).max() == 0.0  # a corner
+
+
 # ---------------------------------------------------------------- the MLP ---
 
 def _mlp_data(n, seed):
@@ -224,7 +270,7 @@ def test_gpu_filter3d(shape_in, shape_out):
     x = rng.standard_normal(shape_in)
     print("run filter3d %s -> %s" % (shape_in, shape_out), flush=True)
     y = mf.to_host(mf.filter3d(*fs, mf.to_device(x)))
-    ref = np.einsum("ai,bj,ck,ijk->abc", Fs[0], Fs[1], Fs[2], x)
+    ref = np.einsum("ai,bj,ck,ijk->abc", Fs[0], Fs[1], Fs[2], x, optimize=True)
     assert y.shape == shape_out
     assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
 
@@ -312,3 +358,92 @@ def test_gpu_mlp_adam_matches_oracle_and_trains():
         loss = net.adam_step(Xd, Td, lr)
     print("loss %.4e -> %.4e" % (first, loss), flush=True)
     assert loss < 0.1 * first
+
+
+@pytest.mark.gpu
+def test_gpu_sgs_targets_and_features():
+    mf, torch = _mf()
+    shape_in, shape_out, nodes = (40, 36, 30), (12, 11, 10), (11, 10, 9)
+    Fs, axes = _filters(shape_in, shape_out, nodes)
+    fs = [mf.SplineFilter(*a, 3) for a in axes]
+    rng = np.random.default_rng(17)
+    q = [rng.standard_normal(shape_in) for _ in range(3)]
+    print("run sgs targets %s -> %s" % (shape_in, shape_out), flush=True)
+    bars, tau = mf.sgs_targets(*fs, *[mf.to_device(a) for a in q], tscale=0.5)
+    qb, tref = L.sgs_targets(*Fs, *q, tscale=0.5)
+    for a, b in zip(bars, qb):
+        assert np.abs(mf.to_host(a) - b).max() < 1e-12 * np.abs(b).max()
+    tau = tau.cpu().numpy()
+    assert tau.shape == tref.shape
+    assert np.abs(tau - tref).max() < 1e-6 * np.abs(tref).max()
+    # features of the filtered field on the output grid
+    grids = [a[1] for a in axes]
+    dcs = [mf.DerivativeCoefficients(g, solver=True) for g in grids]
+    print("run sgs features", flush=True)
+    X = mf.sgs_features(dcs, *bars, grids, gscale=0.2, lscale=7.0).cpu().numpy()
+    import mf_oracle as o
+    odc = [o.DerCoeffs(g, solver=True, so=6) for g in grids]
+    du = [[o.derivative_3d(odc[j], qb[i], j, 1) for j in range(3)] for i in range(3)]
+    Xr = L.sgs_features(du, *grids, 0.2, 7.0)
+    assert np.abs(X - Xr).max() < 1e-6 * np.abs(Xr).max()
+
+
+@pytest.mark.gpu
+def test_gpu_ns_learned_closure_matches_oracle():
+    import ns_oracle as ns
+    from mflops import synthetic
+    mf, torch = _mf()
+    gx, gy, gz, u, v, w, bc = synthetic.channel_problem(20, 24, 18, amp=0.1)
+    re, dt, steps = 3000.0, 1e-3, 3
+    p = L.init_params(44)
+    p[L.NPARAM - 6:] = 0.02
+    scales = dict(gscale=0.1, lscale=10.0, tscale=1e-2)
+    orc = ns.NSOracle(gx, gy, gz, re, dt)
+    orc.set_state(u, v, w)
+    for c in range(3):
+        orc.bc["uvw"[c]] = bc[c]
+    orc.set_sgs(p, scales["gscale"], scales["lscale"], scales["tscale"])
+    for _ in range(steps):
+        orc.step()
+    print("run ns learned closure", flush=True)
+    runs = []
+    for with_sgs in (True, False):
+        dev = mf.NavierStokes(gx, gy, gz, re, dt)
+        for c in range(3):
+            dev.set_bc(c, mf.BoundaryCondition(*[mf.to_device(b) for b in bc[c]]))
+        dev.set_state(*[mf.to_device(f) for f in (u, v, w)])
+        net = mf.SGSNet(p)
+        if with_sgs:
+            dev.set_sgs(net, **scales)
+        dev.step(steps, mf.MF_FP64)
+        runs.append([mf.to_host(t) for t in dev.get()[:4]])
+    want = list(orc.velocity()) + [orc.pressure()]
+    for name, a, b, c in zip("uvwp", runs[0], want, runs[1]):
+        scale = np.abs(b).max()
+        err, effect = np.abs(a - b).max() / scale, np.abs(c - b).max() / scale
+        print("  %s: gpu vs oracle %.2e, closure effect %.2e" % (name, err, effect), flush=True)
+        # the closure runs in FP32 with bf16 weights on the device and in
+        # FP64 with the same weights in the oracle
+        assert err < 1e-6 and effect > 100 * err
+
+
+@pytest.mark.gpu
+def test_gpu_sgs_workflow():
+    """Fine run -> filtered snapshots -> trained closure -> coarse runs with
+    no model, Smagorinsky and the closure against the filtered fine run.
+    The check is that every stage runs on the device and behaves: training
+    lowers the loss, every coarse run stays bounded.  Which model is closest
+    is what the workflow measures, not a property to assert."""
+    mf, torch = _mf()
+    from mflops import sgs
+    print("run sgs workflow", flush=True)
+    # wavenumbers up to 2: the coarse grid resolves the initial field
+    res = sgs.run(mf, fine_shape=(40, 48, 32), coarse_shape=(20, 24, 16), steps=30,
+                  snap_every=10, train_steps=150, kmax=2, log=lambda m: print(m, flush=True))
+    assert res.samples > 3 * (18 * 22 * 14)
+    assert res.loss[1] < 0.9 * res.loss[0]
+    for model, errs in res.errors.items():
+        assert all(np.isfinite(e) and e < 0.5 for e in errs), (model, errs)
+    # the eddy viscosity and the closure change the coarse solution
+    assert res.errors["smagorinsky"] != res.errors["none"]
+    assert res.errors["learned"] != res.errors["none"]
