@@ -466,7 +466,7 @@ extern "C" int mf_filter_create(const double *xi, const double *wi, int nin, con
       if (!(wi[r] > 0)) return mf::fail(MF_ERR_ARG, "mf_filter_create: weights must be positive");
   if (xn[0] != xi[0] || xn[nn - 1] != xi[nin - 1])
     return mf::fail(MF_ERR_ARG, "mf_filter_create: the nodes must start and end at the input end points");
-  // knots (class_filter.f90:165-169)
+  // knots (class_filter.f90:149-154)
   std::vector<long double> t;
   for (int q = 0; q <= k; ++q) t.push_back(xi[0]);
   for (int q = 1; q < nn - 1; ++q) t.push_back(xn[q]);

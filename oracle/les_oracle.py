@@ -39,7 +39,7 @@ import numpy as np
 # ----------------------------------------------------------------------------
 
 def knots(xb: float, xe: float, xn, k: int) -> np.ndarray:
-    """curfit iopt = -1 knots (class_filter.f90:165-169)."""
+    """curfit iopt = -1 knots (class_filter.f90:149-154)."""
     xn = np.asarray(xn, dtype=np.float64)
     return np.concatenate([[xb] * (k + 1), xn[1:-1], [xe] * (k + 1)])
 
