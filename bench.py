@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "mflops3d-md_amd"))
 
 METRIC = "Helmholtz solves/s @256³ mixed vs FP64 at ≤1e-12 rel residual; NS steps/s"
 PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0,
-        "hbm_gbs": 8000.0}
+        "fp16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
 COPY_PROBE_BYTES = 256 << 20
 # HBM bytes per launch of the transform kernels, from the FETCH_SIZE / WRITE_SIZE
 # PMC runs (scaled by the copy probe); written by the profiling step
@@ -122,7 +122,8 @@ def main():
             rep = sc.solve(gd, u, bc, sigma, prec, tol=args.tol)
         return rep
 
-    variants = (("fp32", mf.MF_MIXED), ("bf16x3", mf.MF_MIXED_BF16X3))
+    variants = (("fp32", mf.MF_MIXED), ("bf16x3", mf.MF_MIXED_BF16X3),
+                ("fp16x2", mf.MF_MIXED_FP16X2))
     umx = {name: mf.fortran_zeros((n, n, n)) for name, _ in variants}
     log("warmup")
     run(mf.MF_FP64, u64, args.warmup)
@@ -169,7 +170,8 @@ def main():
     err_mx_64 = {name: float(np.abs(a[inner] - a64[inner]).max() / scale) for name, a in amx.items()}
 
     def roof(p, key, peak, ops=1.0, traf=None):
-        """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3); traf:
+        """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3, 3 for
+        fp16x2); traf:
         measured HBM bytes per launch (PMC), or None."""
         ms, cnt, work = p[key]
         if cnt == 0:
@@ -184,7 +186,9 @@ def main():
     roofs = {"fp32": roof(res["fp32"][2], "transform_fp32", PEAK["fp32_mfma_tflops"],
                           traf=measured_traffic("gemm_mode_kernel<float")),
              "bf16x3": roof(res["bf16x3"][2], "transform_fp32", PEAK["bf16_mfma_tflops"], 6.0,
-                            traf=measured_traffic("gemm_bf3_kernel"))}
+                            traf=measured_traffic("gemm_bf3_kernel")),
+             "fp16x2": roof(res["fp16x2"][2], "transform_fp32", PEAK["fp16_mfma_tflops"], 3.0,
+                            traf=measured_traffic("gemm_h2_kernel"))}
     r64 = roof(prof64, "transform_fp64", PEAK["fp64_mfma_tflops"],
                traf=measured_traffic("gemm_mode_kernel<double"))
     ms_r, cnt_r, work_r = profmx["residual_fp64"]
@@ -195,7 +199,8 @@ def main():
                 for k, v in p.items() if v[1]}
 
     dtypes = {"fp32": "f32 MFMA transforms + f64 residual",
-              "bf16x3": "3-part bf16 MFMA transforms (f32 accumulate) + f64 residual"}
+              "bf16x3": "3-part bf16 MFMA transforms (f32 accumulate) + f64 residual",
+              "fp16x2": "scaled 2-part fp16 MFMA transforms (f32 accumulate) + f64 residual"}
     out = {
         "metric": METRIC,
         "value": round(1.0 / tmx, 3),

@@ -44,9 +44,11 @@ enum {
 
 /* MF_MIXED: FP32 MFMA transforms.  MF_MIXED_BF16X3: the same transforms with
  * every operand split into three bf16 parts (six bf16 MFMA products, FP32
- * accumulation: FP32-class accuracy on the bf16 matrix cores).  Both refine
- * with the FP64 residual to tol. */
-enum { MF_FP64 = 0, MF_MIXED = 1, MF_MIXED_BF16X3 = 2 };
+ * accumulation: FP32-class accuracy on the bf16 matrix cores).
+ * MF_MIXED_FP16X2: operands scaled by powers of two and split into two fp16
+ * parts (three fp16 MFMA products: 22-bit accuracy at half the matrix-core
+ * work of BF16X3).  All refine with the FP64 residual to tol. */
+enum { MF_FP64 = 0, MF_MIXED = 1, MF_MIXED_BF16X3 = 2, MF_MIXED_FP16X2 = 3 };
 
 typedef struct mf_der mf_der;
 typedef struct mf_solver3d mf_solver3d;

@@ -52,6 +52,9 @@ struct ResArgs {
   const double *fbx, *fby, *fbz;  // face products (n2,n3,2), (n1,n3,2), (n1,n2,2)
   float *out;            // r', interior-contiguous
   double *part;          // per-block partial sums of r'^2
+  unsigned *amax;        // NULL or: max |r'| (float bits, atomic max per block)
+  unsigned *zero;        // NULL or: nzero scalars block 0 sets to zero
+  int nzero;
 };
 
 // RES = false: right-hand side b' = P g_f (u is not read);
@@ -253,6 +256,7 @@ __global__ void __launch_bounds__(64 * RZ_NW) residual_kernel(ResArgs a) {
   }
   // 4. boundary fold, store r' (FP32), norm partials
   double contrib = 0.0;
+  float vmax = 0.0f;
   if (xout) {
     const double ex1 = a.ex[i], ex2 = a.ex[n1 + i];
     const long sx = (long)n2 * n3, sy = (long)n1 * n3, sz = (long)n1 * n2;
@@ -275,17 +279,31 @@ __global__ void __launch_bounds__(64 * RZ_NW) residual_kernel(ResArgs a) {
         if (ez2 != 0.0) v = fma(-ez2, a.fbz[sz + i + (long)n1 * j], v);
         a.out[i + (long)n1 * (j + (long)n2 * k)] = (float)v;
         contrib = fma(v, v, contrib);
+        vmax = fmaxf(vmax, fabsf((float)v));
       }
     }
   }
   __shared__ double wsum[RZ_NW];
+  __shared__ float wmax[RZ_NW];
   for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
-  if (lane == 0) wsum[w] = contrib;
+  if (a.amax)
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+  if (lane == 0) {
+    wsum[w] = contrib;
+    wmax[w] = vmax;
+  }
   __syncthreads();
   if (tid == 0) {
     double t = 0.0;
     for (int q = 0; q < RZ_NW; ++q) t += wsum[q];
     a.part[blockIdx.x] = t;
+    if (a.amax) {
+      float b = wmax[0];
+      for (int q = 1; q < RZ_NW; ++q) b = fmaxf(b, wmax[q]);
+      atomicMax(a.amax, __float_as_uint(b));
+    }
+    if (a.zero && blockIdx.x == 0)
+      for (int q = 0; q < a.nzero; ++q) a.zero[q] = 0u;
   }
 }
 
@@ -298,10 +316,12 @@ struct FaceFoldArgs {
   BandRows Mx, My, Mz;
   const double *bcx, *bcy, *bcz;
   double *fbx, *fby, *fbz;
+  unsigned *zero;  // NULL or a scalar thread 0 sets to zero
 };
 
 __global__ void __launch_bounds__(256) face_fold_kernel(FaceFoldArgs f) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f.zero && t == 0) *f.zero = 0u;
   const long sx = (long)f.n2 * f.n3, sy = (long)f.n1 * f.n3, sz = (long)f.n1 * f.n2;
   const double *src;
   double *dst;

@@ -26,6 +26,7 @@
 
 #include "gemm.h"
 #include "gemm_bf3.h"
+#include "gemm_h2.h"
 #include "influence.h"
 #include "internal.h"
 #include "residual.h"
@@ -149,6 +150,13 @@ struct mf_solver3d {
   // (row = the eigenvector-dimension index, zero-padded to wrows[axis] rows
   // and kp[axis] columns)
   __bf16 *Wsp[3] = {nullptr, nullptr, nullptr}, *Vsp[3] = {nullptr, nullptr, nullptr};
+  // and into two fp16 parts of the matrix scaled by wsc (a power of two),
+  // same layout
+  _Float16 *Whs[3] = {nullptr, nullptr, nullptr}, *Vhs[3] = {nullptr, nullptr, nullptr};
+  float wsc[3][2] = {{1, 1}, {1, 1}, {1, 1}};
+  // largest magnitudes (float bits) of the fp16 path's fields: [0] r' / b',
+  // [1..5] the outputs of the first five transforms
+  unsigned *fmx = nullptr;
   int kp[3] = {0, 0, 0}, wrows[3] = {0, 0, 0};
   double *lam = nullptr;   // lx(n1) ly(n2) lz(n3)
   double *fold = nullptr;  // fx1 fx2 fy1 fy2 fz1 fz2
@@ -469,7 +477,11 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
       const size_t cnt = (3 * (size_t)h->wrows[a] * h->kp[a] + 3) / 4;  // bf16 -> doubles
       h->Wsp[a] = (__bf16 *)take(cnt);
       h->Vsp[a] = (__bf16 *)take(cnt);
+      const size_t cnh = (2 * (size_t)h->wrows[a] * h->kp[a] + 3) / 4;  // fp16 -> doubles
+      h->Whs[a] = (_Float16 *)take(cnh);
+      h->Vhs[a] = (_Float16 *)take(cnh);
     }
+    h->fmx = (unsigned *)take(4);
     return off;
   };
   const size_t nd = carve(nullptr);
@@ -513,6 +525,28 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
             sp[(((size_t)q * (kp / 8) + k / 8) * wr + r) * 8 + k % 8] = p[q];
         }
       e = up(which == 0 ? (void *)h->Wsp[a] : (void *)h->Vsp[a], sp.data(), sp.size() * 2);
+      // two fp16 parts of the matrix scaled so that its largest entry lands
+      // in [2^13, 2^14)
+      double big = 0.0;
+      for (size_t q = 0; q < (size_t)m * m; ++q) big = std::max(big, std::fabs(src[q]));
+      int ex = 0;
+      (void)std::frexp(big, &ex);  // big in [2^(ex-1), 2^ex)
+      const float sc = big > 0.0 ? (float)std::ldexp(1.0, 14 - ex) : 1.0f;
+      h->wsc[a][which] = sc;
+      std::vector<uint16_t> hp((size_t)2 * wr * kp, 0);
+      for (int r = 0; r < m; ++r)
+        for (int k = 0; k < m; ++k) {
+          const double v = (a == 0) ? src[r + (size_t)m * k] : src[k + (size_t)m * r];
+          const float xs = (float)v * sc;
+          const _Float16 h0 = (_Float16)xs, h1 = (_Float16)(xs - (float)h0);
+          uint16_t b0, b1;
+          std::memcpy(&b0, &h0, 2);
+          std::memcpy(&b1, &h1, 2);
+          hp[(((size_t)0 * (kp / 8) + k / 8) * wr + r) * 8 + k % 8] = b0;
+          hp[(((size_t)1 * (kp / 8) + k / 8) * wr + r) * 8 + k % 8] = b1;
+        }
+      if (e == hipSuccess)
+        e = up(which == 0 ? (void *)h->Whs[a] : (void *)h->Vhs[a], hp.data(), hp.size() * 2);
     }
   }
   if (e == hipSuccess) e = up(h->lam, lam.data(), lam.size() * sizeof(double));
@@ -679,6 +713,73 @@ void transforms_bf3(mf_solver3d *h, const float *in, float *w1, float *w2, doubl
     launch_bf3<float, double, true, mf::EPI_SCATTER>(h, w1, n1, 0, h->Vsp[0], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
 }
 
+// ---- two-part fp16 transforms (MF_MIXED_FP16X2)
+template <typename TF, typename TC, int BM, int BN, bool WLEFT, int EPI>
+void launch_h2_tile(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W, float wsc,
+                    int KP, int WROWS, TC *C, long ldc, long sC, int M, int N, int K, int batch,
+                    const EpiArgs &ep, int slot_in, int slot_out, bool zero0, hipStream_t st) {
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  ProfScope ps(h, P_T32, 2.0 * M * N * K * batch, st);
+  hipLaunchKernelGGL((mf::gemm_h2_kernel<TF, TC, BM, BN, WLEFT, EPI>), grid, dim3(256), 0, st, F,
+                     ldf, sF, W, KP, WROWS, C, ldc, sC, M, N, K, ep,
+                     (const unsigned *)(h->fmx + slot_in), wsc,
+                     slot_out >= 0 ? h->fmx + slot_out : (unsigned *)nullptr,
+                     zero0 ? h->fmx : (unsigned *)nullptr);
+}
+
+template <typename TF, typename TC, bool WLEFT, int EPI>
+void launch_h2(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W, float wsc,
+               int KP, int WROWS, TC *C, long ldc, long sC, int M, int N, int K, int batch,
+               const EpiArgs &ep, int slot_in, int slot_out, bool zero0, hipStream_t st) {
+  const int t = gemm_tile(WLEFT ? M : N);
+#define MF_H2(BM, BN)                                                                      \
+  launch_h2_tile<TF, TC, BM, BN, WLEFT, EPI>(h, F, ldf, sF, W, wsc, KP, WROWS, C, ldc, sC, M, N, \
+                                             K, batch, ep, slot_in, slot_out, zero0, st)
+  if (WLEFT) {
+    if (t == 64) MF_H2(64, 256);
+    else if (t == 128) MF_H2(128, 128);
+    else MF_H2(256, 64);
+  } else {
+    if (t == 64) MF_H2(256, 64);
+    else if (t == 128) MF_H2(128, 128);
+    else MF_H2(64, 256);
+  }
+#undef MF_H2
+}
+
+// The six transforms of transforms_bf3 on the two-part fp16 matrices.  Field
+// scales: slot 0 holds max |r'| (written by the residual kernel), slot q the
+// largest output of transform q; the last transform clears slot 0 for the
+// residual that follows (the residual clears slots 1-5).
+void transforms_h2(mf_solver3d *h, const float *in, float *w1, float *w2, double sigma,
+                   double *u, bool accumulate, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  const long n12 = (long)n1 * n2;
+  const int kx = h->kp[0], ky = h->kp[1], kz = h->kp[2];
+  const int rx = h->wrows[0], ry = h->wrows[1], rz = h->wrows[2];
+  EpiArgs ep;
+  std::memset(&ep, 0, sizeof(ep));
+  ep.lx = h->lam;
+  ep.ly = h->lam + n1;
+  ep.lz = h->lam + n1 + n2;
+  ep.sigma = sigma;
+  ep.n1 = n1;
+  ep.n2 = n2;
+  ep.u = u;
+  ep.nx = h->nx;
+  ep.ny = h->ny;
+  using mf::EPI_STORE;
+  launch_h2<float, float, true, EPI_STORE>(h, in, n1, 0, h->Whs[0], h->wsc[0][0], kx, rx, w1, n1, 0, n1, n2 * n3, n1, 1, ep, 0, 1, false, st);
+  launch_h2<float, float, false, EPI_STORE>(h, w1, n1, n12, h->Whs[1], h->wsc[1][0], ky, ry, w2, n1, n12, n1, n2, n2, n3, ep, 1, 2, false, st);
+  launch_h2<float, float, false, mf::EPI_DIVIDE>(h, w2, n12, 0, h->Whs[2], h->wsc[2][0], kz, rz, w1, n12, 0, (int)n12, n3, n3, 1, ep, 2, 3, false, st);
+  launch_h2<float, float, false, EPI_STORE>(h, w1, n12, 0, h->Vhs[2], h->wsc[2][1], kz, rz, w2, n12, 0, (int)n12, n3, n3, 1, ep, 3, 4, false, st);
+  launch_h2<float, float, false, EPI_STORE>(h, w2, n1, n12, h->Vhs[1], h->wsc[1][1], ky, ry, w1, n1, n12, n1, n2, n2, n3, ep, 4, 5, false, st);
+  if (accumulate)
+    launch_h2<float, double, true, mf::EPI_SCATTER_ADD>(h, w1, n1, 0, h->Vhs[0], h->wsc[0][1], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, 5, -1, true, st);
+  else
+    launch_h2<float, double, true, mf::EPI_SCATTER>(h, w1, n1, 0, h->Vhs[0], h->wsc[0][1], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, 5, -1, true, st);
+}
+
 int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy,
                  const double *bcz, hipStream_t st) {
   const long nf = 2 * ((long)h->n2 * h->n3 + (long)h->n1 * h->n3 + (long)h->n1 * h->n2);
@@ -726,17 +827,27 @@ int launch_residual(mf_solver3d *h, const mf::ResArgs &a, double *norm2_dev, hip
 
 // r' = P g_f - B u (FP32 copy -> R32) and ||r'||^2 -> norm2_dev, in one pass
 // over u and g; the face products must be current (rhs_prime).
+// scales: max |r'| into slot 0 of the fp16 path (cleared by the transforms
+// before), slots 1-5 cleared for the transforms after
+void with_scales(mf_solver3d *h, mf::ResArgs &a) {
+  a.amax = h->fmx;
+  a.zero = h->fmx + 1;
+  a.nzero = 5;
+}
+
 int residual(mf_solver3d *h, const double *u, const double *g, double sigma, double *norm2_dev,
-             hipStream_t st) {
+             bool scales, hipStream_t st) {
   const long N = (long)h->n1 * h->n2 * h->n3;
   ProfScope ps(h, P_RES, 20.0 * N, st);  // algorithmic: read u and g, write r' (FP32)
-  return launch_residual<true>(h, res_args(h, u, g, sigma), norm2_dev, st);
+  mf::ResArgs a = res_args(h, u, g, sigma);
+  if (scales) with_scales(h, a);
+  return launch_residual<true>(h, a, norm2_dev, st);
 }
 
 // b' = P g_f (FP32 copy -> R32) with ||b'||^2, and the face products of the
 // boundary fold for the residuals that follow.
 int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *bcy,
-              const double *bcz, double *norm2_dev, hipStream_t st) {
+              const double *bcz, double *norm2_dev, bool scales, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
   ProfScope ps(h, P_MULT, 12.0 * N, st);  // algorithmic: read g, write b' (FP32)
@@ -747,9 +858,12 @@ int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *
   f.fbx = h->fbf;
   f.fby = f.fbx + 2L * n2 * n3;
   f.fbz = f.fby + 2L * n1 * n3;
+  f.zero = scales ? h->fmx : nullptr;
   const long nf = 2 * ((long)n2 * n3 + (long)n1 * n3 + (long)n1 * n2);
   hipLaunchKernelGGL(mf::face_fold_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, f);
-  return launch_residual<false>(h, res_args(h, nullptr, g, 0.0), norm2_dev, st);
+  mf::ResArgs a = res_args(h, nullptr, g, 0.0);
+  if (scales) with_scales(h, a);
+  return launch_residual<false>(h, a, norm2_dev, st);
 }
 }  // namespace
 
@@ -759,9 +873,10 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
                                  void *stream) {
   if (!h || !g || !u || !bcx || !bcy || !bcz)
     return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: null pointer");
-  if (precision != MF_FP64 && precision != MF_MIXED && precision != MF_MIXED_BF16X3)
-    return mf::fail(MF_ERR_ARG,
-                    "mf_solver3d_solve: precision must be MF_FP64, MF_MIXED or MF_MIXED_BF16X3");
+  if (precision != MF_FP64 && precision != MF_MIXED && precision != MF_MIXED_BF16X3 &&
+      precision != MF_MIXED_FP16X2)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: precision must be MF_FP64, MF_MIXED, "
+                                "MF_MIXED_BF16X3 or MF_MIXED_FP16X2");
   if (g == u) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: g and u must not alias");
   if (!std::isfinite(sigma)) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: sigma is not finite");
   hipStream_t st = (hipStream_t)stream;
@@ -784,16 +899,19 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   auto inner = [&](const float *in, bool accumulate) {
     if (precision == MF_MIXED_BF16X3)
       transforms_bf3(h, in, h->F1, h->F2, sigma, u, accumulate, st);
+    else if (precision == MF_MIXED_FP16X2)
+      transforms_h2(h, in, h->F1, h->F2, sigma, u, accumulate, st);
     else
       transforms<float, float>(h, in, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
                                h->Vyt32, h->Vzt32, sigma, u, accumulate, st);
   };
   if (max_iter > MF_MAX_REFINE) max_iter = MF_MAX_REFINE;
-  int rc = rhs_prime(h, g, bcx, bcy, bcz, h->norms + 0, st);
+  const bool scales = precision == MF_MIXED_FP16X2;
+  int rc = rhs_prime(h, g, bcx, bcy, bcz, h->norms + 0, scales, st);
   if (rc != MF_OK) return rc;
   // first low-precision solve: u_int = S(b'); norms[it] = ||r'||^2 after it solves
   inner(h->R32, false);
-  rc = residual(h, u, g, sigma, h->norms + 1, st);
+  rc = residual(h, u, g, sigma, h->norms + 1, scales, st);
   if (rc != MF_OK) return rc;
   int it = 1;
   // One FP32-class solve leaves a residual near 1e-7, so below that
@@ -801,7 +919,7 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   // first residual (one host round trip less).
   if (tol < 1e-6 && max_iter >= 2) {
     inner(h->R32, true);
-    rc = residual(h, u, g, sigma, h->norms + 2, st);
+    rc = residual(h, u, g, sigma, h->norms + 2, scales, st);
     if (rc != MF_OK) return rc;
     it = 2;
   }
@@ -820,7 +938,7 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
     if (!(rel > tol) || it >= max_iter) break;
     inner(h->R32, true);
     ++it;
-    rc = residual(h, u, g, sigma, h->norms + it, st);
+    rc = residual(h, u, g, sigma, h->norms + it, scales, st);
     if (rc != MF_OK) return rc;
   }
   rc = launch_faces(h, u, bcx, bcy, bcz, st);

@@ -17,7 +17,8 @@
 module mflops_c
   use iso_c_binding
   implicit none
-  integer(c_int), parameter :: MF_OK = 0, MF_FP64 = 0, MF_MIXED = 1, MF_MIXED_BF16X3 = 2
+  integer(c_int), parameter :: MF_OK = 0, MF_FP64 = 0, MF_MIXED = 1, MF_MIXED_BF16X3 = 2, &
+                                MF_MIXED_FP16X2 = 3
 
   type, bind(c) :: mf_solve_report
      integer(c_int) :: iterations
@@ -273,7 +274,7 @@ module mflops_solver_3d
   end type solver_coeffs_3d
 
   public :: solver_coeffs_init_3d, solve_3d, solver_3d_array, solver_coeffs_destroy_3d
-  public :: MF_FP64, MF_MIXED, MF_MIXED_BF16X3, mf_solve_report
+  public :: MF_FP64, MF_MIXED, MF_MIXED_BF16X3, MF_MIXED_FP16X2, mf_solve_report
 
 contains
 

@@ -23,10 +23,11 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import MF_FP64, MF_MIXED, MF_MIXED_BF16X3, MFError, SolveReport, check, load  # noqa: F401
+from ._lib import (MF_FP64, MF_MIXED, MF_MIXED_BF16X3, MF_MIXED_FP16X2, MFError,  # noqa: F401
+                   SolveReport, check, load)
 
 __all__ = [
-    "load", "MFError", "MF_FP64", "MF_MIXED", "MF_MIXED_BF16X3", "SolveReport",
+    "load", "MFError", "MF_FP64", "MF_MIXED", "MF_MIXED_BF16X3", "MF_MIXED_FP16X2", "SolveReport",
     "fortran_empty", "fortran_zeros", "to_device", "to_host", "is_fortran",
     "DerivativeCoefficients", "derivative", "derx", "dderx", "dery", "ddery", "derz", "dderz",
     "BoundaryCondition", "SolverCoeffs3D", "solver_3d", "der_coeffs_1d", "solver_coeffs_1d",

@@ -8,7 +8,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmflops.so")
 
 MF_OK, MF_ERR_ARG, MF_ERR_SETUP, MF_ERR_DEVICE, MF_ERR_NOCONV = 0, 1, 2, 3, 4
-MF_FP64, MF_MIXED, MF_MIXED_BF16X3 = 0, 1, 2
+MF_FP64, MF_MIXED, MF_MIXED_BF16X3, MF_MIXED_FP16X2 = 0, 1, 2, 3
 
 # every exported symbol with (restype, argtypes)
 _dp = C.POINTER(C.c_double)
