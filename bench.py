@@ -7,14 +7,16 @@ and z, homogeneous Dirichlet walls; RHS = 8 seeded random-phase cos modes
 (wavenumbers 1-16) + 1e-3 white noise.  sigma is the prediction step's
 Helmholtz coefficient -Re*fac(1) (BDF2, Re=3000, dt=1e-3).  A step is one
 complete solve g -> u with inputs resident in HBM.  The mixed solves (FP32
-MFMA or three-part bf16 MFMA transforms, FP64-residual refinement) iterate to
+MFMA, three-part bf16 or scaled two-part fp16 MFMA transforms, FP64-residual
+refinement) iterate to
 a relative residual <= 1e-12; the FP64 solve is the reference algorithm in
 FP64 on MFMA.
 
 Prints one JSON line (value = the faster mixed variant's solves/s) with the
-roofline of its dominant kernel (the transform GEMM; average launch time from
-HIP events recorded around every kernel in a second pass over the same solves,
-right after the timed pass, since the event records add gaps between kernels)
+roofline of its dominant kernel (the transform GEMM; its average launch time
+from the first five transforms of a solve launched back to back between two
+HIP events; per-launch event pairs, used only for the breakdown, add the
+host's enqueue gaps)
 and the CPU baseline (the FP64 oracle on the host's cores, a bounded sample of
 the same workload).
 """
@@ -169,28 +171,41 @@ def main():
     scale = float(np.abs(a64[inner]).max())
     err_mx_64 = {name: float(np.abs(a[inner] - a64[inner]).max() / scale) for name, a in amx.items()}
 
-    def roof(p, key, peak, ops=1.0, traf=None):
-        """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3, 3 for
-        fp16x2); traf:
-        measured HBM bytes per launch (PMC), or None."""
-        ms, cnt, work = p[key]
-        if cnt == 0:
-            return None
-        avg = ms / cnt * 1e-3
-        ach = ops * work / cnt / avg / 1e12
-        return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 4), "traffic": traf,
-                "launches": cnt, "avg_launch_ms": round(ms / cnt, 4),
-                "flops_per_launch": ops * work / cnt}
+    # the transform kernels' own durations: the first five transforms of a
+    # solve launched back to back (per-launch event pairs would add the
+    # host's enqueue gaps)
+    kt = {name: sc.time_transforms(prec, reps=10) for name, prec in
+          (("fp64", mf.MF_FP64),) + variants}
 
-    roofs = {"fp32": roof(res["fp32"][2], "transform_fp32", PEAK["fp32_mfma_tflops"],
+    def roof(name, peak, ops=1.0, traf=None, bytes_per_point=8.0):
+        """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3, 3 for
+        fp16x2); traf: measured HBM bytes per launch (PMC), or None.  The
+        bound is the slower of the two roofs: MFMA at peak, or HBM for one
+        read and one write of the field (bytes_per_point in all)."""
+        ms, flops = kt[name]
+        avg = ms * 1e-3
+        npts = (n - 2) ** 3
+        t_mfma = ops * flops / (peak * 1e12)
+        t_hbm = bytes_per_point * npts / (PEAK["hbm_gbs"] * 1e9)
+        common = {"traffic": traf, "avg_launch_ms": round(ms, 4), "flops_per_launch": flops,
+                  "bytes_per_launch": bytes_per_point * npts,
+                  "mfma_frac": round(t_mfma / avg, 4), "hbm_frac": round(t_hbm / avg, 4)}
+        if t_mfma >= t_hbm:
+            ach = ops * flops / avg / 1e12
+            return dict({"bound": "mfma", "achieved": round(ach, 2), "peak": peak,
+                         "unit": "TFLOP/s", "frac": round(ach / peak, 4)}, **common)
+        ach = bytes_per_point * npts / avg / 1e9
+        return dict({"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK["hbm_gbs"],
+                     "unit": "GB/s", "frac": round(ach / PEAK["hbm_gbs"], 4)}, **common)
+
+    roofs = {"fp32": roof("fp32", PEAK["fp32_mfma_tflops"],
                           traf=measured_traffic("gemm_mode_kernel<float")),
-             "bf16x3": roof(res["bf16x3"][2], "transform_fp32", PEAK["bf16_mfma_tflops"], 6.0,
+             "bf16x3": roof("bf16x3", PEAK["bf16_mfma_tflops"], 6.0,
                             traf=measured_traffic("gemm_bf3_kernel")),
-             "fp16x2": roof(res["fp16x2"][2], "transform_fp32", PEAK["fp16_mfma_tflops"], 3.0,
+             "fp16x2": roof("fp16x2", PEAK["fp16_mfma_tflops"], 3.0,
                             traf=measured_traffic("gemm_h2_kernel"))}
-    r64 = roof(prof64, "transform_fp64", PEAK["fp64_mfma_tflops"],
-               traf=measured_traffic("gemm_mode_kernel<double"))
+    r64 = roof("fp64", PEAK["fp64_mfma_tflops"], traf=measured_traffic("gemm_mode_kernel<double"),
+               bytes_per_point=16.0)
     ms_r, cnt_r, work_r = profmx["residual_fp64"]
     res_gbs = (work_r / cnt_r) / (ms_r / cnt_r * 1e-3) / 1e9 if cnt_r else None
 
@@ -247,8 +262,9 @@ def main():
 def ns_segment(args, cpu_steps=0):
     """NS prediction-projection steps/s on the 192x256x192 channel (Re=3000,
     dt=1e-3, tanh stretch 2.5 in y): args.ns_warmup untimed then
-    args.ns_steps timed steps, for FP64 velocity solves and for the split-bf16
-    mixed velocity solves (tol 1e-12); the pressure solve is FP64 in both."""
+    args.ns_steps timed steps, for FP64 velocity solves and for the scaled
+    two-part fp16 mixed velocity solves (tol 1e-12); the pressure solve is FP64
+    in both."""
     import numpy as np
     import torch
 
@@ -267,7 +283,7 @@ def ns_segment(args, cpu_steps=0):
     fields = [mf.to_device(f) for f in (u, v, w)]
     bcs = [mf.BoundaryCondition(*[mf.to_device(b) for b in bc[c]]) for c in range(3)]
     finals = {}
-    for name, prec in (("fp64", mf.MF_FP64), ("mixed_bf16x3", mf.MF_MIXED_BF16X3)):
+    for name, prec in (("fp64", mf.MF_FP64), ("mixed_fp16x2", mf.MF_MIXED_FP16X2)):
         s = mf.NavierStokes(gx, gy, gz, re, dt)
         for c in range(3):
             s.set_bc(c, bcs[c])
@@ -284,9 +300,9 @@ def ns_segment(args, cpu_steps=0):
         finals[name] = mf.to_host(s.get()[0])
         log("ns %s: %.3f ms/step" % (name, ts * 1e3))
         del s
-    a, b = finals["fp64"], finals["mixed_bf16x3"]
+    a, b = finals["fp64"], finals["mixed_fp16x2"]
     res["u_mixed_vs_fp64"] = float(np.abs(a - b).max() / np.abs(a).max())
-    res["value"] = res["mixed_bf16x3"]
+    res["value"] = res["mixed_fp16x2"]
     return res
 
 

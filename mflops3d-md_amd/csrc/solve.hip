@@ -616,7 +616,7 @@ void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, lo
 template <typename T, typename TIn>
 void transforms(mf_solver3d *h, const TIn *in, T *w1, T *w2, const T *Wx, const T *Wyt,
                 const T *Wzt, const T *Vx, const T *Vyt, const T *Vzt, double sigma, double *u,
-                bool accumulate, hipStream_t st) {
+                bool accumulate, hipStream_t st, bool last = true) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long n12 = (long)n1 * n2;
   EpiArgs ep;
@@ -641,6 +641,7 @@ void transforms(mf_solver3d *h, const TIn *in, T *w1, T *w2, const T *Wx, const 
   // backward Y: w1_k = w2_k . Vyt
   launch_gemm<T, T, T, T, false, mf::EPI_STORE>(h, w2, n1, n12, Vyt, n2, w1, n1, n12, n1, n2, n2, n3, ep, st);
   // backward X into u's interior
+  if (!last) return;
   if (accumulate)
     launch_gemm<T, T, T, double, true, mf::EPI_SCATTER_ADD>(h, Vx, n1, 0, w1, n1, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
   else
@@ -685,7 +686,7 @@ void launch_bf3(mf_solver3d *h, const TF *F, long ldf, long sF, const __bf16 *W,
 // The six transforms of transforms<float, double> with the pre-split
 // matrices W' = inv(V) inv(lhs) and V.
 void transforms_bf3(mf_solver3d *h, const float *in, float *w1, float *w2, double sigma,
-                    double *u, bool accumulate, hipStream_t st) {
+                    double *u, bool accumulate, hipStream_t st, bool last = true) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long n12 = (long)n1 * n2;
   const int kx = h->kp[0], ky = h->kp[1], kz = h->kp[2];
@@ -707,6 +708,7 @@ void transforms_bf3(mf_solver3d *h, const float *in, float *w1, float *w2, doubl
   launch_bf3<float, float, false, mf::EPI_DIVIDE>(h, w2, n12, 0, h->Wsp[2], kz, rz, w1, n12, 0, (int)n12, n3, n3, 1, ep, st);
   launch_bf3<float, float, false, EPI_STORE>(h, w1, n12, 0, h->Vsp[2], kz, rz, w2, n12, 0, (int)n12, n3, n3, 1, ep, st);
   launch_bf3<float, float, false, EPI_STORE>(h, w2, n1, n12, h->Vsp[1], ky, ry, w1, n1, n12, n1, n2, n2, n3, ep, st);
+  if (!last) return;
   if (accumulate)
     launch_bf3<float, double, true, mf::EPI_SCATTER_ADD>(h, w1, n1, 0, h->Vsp[0], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, st);
   else
@@ -752,7 +754,7 @@ void launch_h2(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W
 // largest output of transform q; the last transform clears slot 0 for the
 // residual that follows (the residual clears slots 1-5).
 void transforms_h2(mf_solver3d *h, const float *in, float *w1, float *w2, double sigma,
-                   double *u, bool accumulate, hipStream_t st) {
+                   double *u, bool accumulate, hipStream_t st, bool last = true) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long n12 = (long)n1 * n2;
   const int kx = h->kp[0], ky = h->kp[1], kz = h->kp[2];
@@ -774,6 +776,7 @@ void transforms_h2(mf_solver3d *h, const float *in, float *w1, float *w2, double
   launch_h2<float, float, false, mf::EPI_DIVIDE>(h, w2, n12, 0, h->Whs[2], h->wsc[2][0], kz, rz, w1, n12, 0, (int)n12, n3, n3, 1, ep, 2, 3, false, st);
   launch_h2<float, float, false, EPI_STORE>(h, w1, n12, 0, h->Vhs[2], h->wsc[2][1], kz, rz, w2, n12, 0, (int)n12, n3, n3, 1, ep, 3, 4, false, st);
   launch_h2<float, float, false, EPI_STORE>(h, w2, n1, n12, h->Vhs[1], h->wsc[1][1], ky, ry, w1, n1, n12, n1, n2, n2, n3, ep, 4, 5, false, st);
+  if (!last) return;
   if (accumulate)
     launch_h2<float, double, true, mf::EPI_SCATTER_ADD>(h, w1, n1, 0, h->Vhs[0], h->wsc[0][1], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, 5, -1, true, st);
   else
@@ -791,6 +794,57 @@ int launch_faces(mf_solver3d *h, double *u, const double *bcx, const double *bcy
 }
 
 mf::BandRows rows(const mf::Band &b) { return mf::BandRows{b.c0, b.w}; }
+}  // namespace
+
+// Back-to-back launches of the first five transforms of a solve (x, y, z
+// with the divide, z and y backward) on the handle's own work arrays, reps
+// times, between two events: the kernels' own durations, without the
+// host-side gaps that per-launch event pairs add.
+extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int reps,
+                                           double *ms_per_launch, double *flops_per_launch,
+                                           void *stream) {
+  if (!h || reps < 1 || !ms_per_launch)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_time_transforms: invalid argument");
+  if (precision < MF_FP64 || precision > MF_MIXED_FP16X2)
+    return mf::fail(MF_ERR_ARG, "mf_solver3d_time_transforms: unknown precision");
+  hipStream_t st = (hipStream_t)stream;
+  const bool prof = h->prof;
+  h->prof = false;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto five = [&] {
+    if (precision == MF_FP64)
+      transforms<double, double>(h, h->T2, h->T1, h->T2, h->Wx, h->Wyt, h->Wzt, h->Vx, h->Vyt,
+                                 h->Vzt, 0.0, nullptr, false, st, false);
+    else if (precision == MF_MIXED)
+      transforms<float, float>(h, h->R32, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
+                               h->Vyt32, h->Vzt32, 0.0, nullptr, false, st, false);
+    else if (precision == MF_MIXED_BF16X3)
+      transforms_bf3(h, h->R32, h->F1, h->F2, 0.0, nullptr, false, st, false);
+    else
+      transforms_h2(h, h->R32, h->F1, h->F2, 0.0, nullptr, false, st, false);
+  };
+  five();  // warm
+  hipError_t e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) five();
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->prof = prof;
+  if (e != hipSuccess)
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_time_transforms: ") + hipGetErrorString(e));
+  *ms_per_launch = ms / (5.0 * reps);
+  if (flops_per_launch)  // 2 M N K per transform, K = n1, n2, n3, n3, n2 in turn
+    *flops_per_launch = 2.0 * h->n1 * h->n2 * h->n3 *
+                        (h->n1 + 2.0 * h->n2 + 2.0 * h->n3) / 5.0;
+  return MF_OK;
+}
+
+namespace {
 
 mf::ResArgs res_args(mf_solver3d *h, const double *u, const double *g, double sigma) {
   mf::ResArgs a;

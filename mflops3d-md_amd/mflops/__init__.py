@@ -285,6 +285,15 @@ class SolverCoeffs3D:
                                                  C.cast(work, C.c_void_p), int(bool(reset))))
         return {k: (ms[i], cnt[i], work[i]) for i, k in enumerate(PROF_CLASSES)}
 
+    def time_transforms(self, precision: int, reps: int = 10, stream=None):
+        """(ms per launch, algorithmic flops per launch) of the first five
+        transforms of a solve launched back to back (overwrites the handle's
+        work arrays only)."""
+        ms, fl = C.c_double(), C.c_double()
+        check(self._lib.mf_solver3d_time_transforms(self._h, int(precision), int(reps),
+                                                    C.byref(ms), C.byref(fl), _stream(stream)))
+        return ms.value, fl.value
+
     def solve(self, g, u, bc: BoundaryCondition, sigma: float, precision: int = MF_FP64,
               tol: float = 1e-12, max_iter: int = 8, stream=None) -> SolveReport:
         """solver_3d(sc, g, u, bc, sigma): u's interior and faces are overwritten."""

@@ -53,6 +53,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_int), C.POINTER(_vp)]),
     "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
     "mf_solver3d_profile_read": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "mf_solver3d_time_transforms": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _vp]),
     "mf_ns_create": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_double, C.c_double,
                                C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "mf_ns_destroy": (C.c_int, [_vp]),
