@@ -443,14 +443,14 @@ int der3(mf_ns *s, int axis, const double *x0, const double *x1, const double *x
 }
 
 // nonlinear terms of the three components at level L (cached): the 9
-// velocity gradients and the 9 product derivatives in 6 batched launches
+// velocity gradients and, for the skew-symmetric form, the 9 product
+// derivatives, one batched launch per axis (6 fields: the line recurrences
+// are latency-bound, so a wider batch costs far less than a second launch)
 int nonlinear(mf_ns *s, int L, hipStream_t st) {
   if (s->nl_ok[L]) return MF_OK;
   const long N = s->N;
   double *u = s->q[0][L], *v = s->q[1][L], *w = s->q[2][L];
   double **T = s->T;  // T[3a + c] gradients, T[9..14] products, T[15 + 3a + c] product derivatives
-  for (int a = 0; a < 3; ++a) RC(der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
-  s->grad_level = L;
   NlArgs n;
   std::memset(&n, 0, sizeof(n));
   n.u = u; n.v = v; n.w = w;
@@ -460,12 +460,20 @@ int nonlinear(mf_ns *s, int L, hipStream_t st) {
     double *uu = T[9], *uv = T[10], *uw = T[11], *vv = T[12], *vw = T[13], *ww = T[14];
     hipLaunchKernelGGL(prod6_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)u,
                        (const double *)v, (const double *)w, uu, uv, uw, vv, vw, ww, N);
-    RC(der3(s, 0, uu, uv, uw, T[15], T[16], T[17], st));  // (q u)_x, q = u, v, w
-    RC(der3(s, 1, uv, vv, vw, T[18], T[19], T[20], st));  // (q v)_y
-    RC(der3(s, 2, uw, vw, ww, T[21], T[22], T[23], st));  // (q w)_z
+    // axis a: d_a of (u, v, w) and of (q u_a), q = u, v, w
+    const double *prod[3][3] = {{uu, uv, uw}, {uv, vv, vw}, {uw, vw, ww}};
+    for (int a = 0; a < 3; ++a) {
+      const double *x[6] = {u, v, w, prod[a][0], prod[a][1], prod[a][2]};
+      double *o[6] = {T[3 * a], T[3 * a + 1], T[3 * a + 2], T[15 + 3 * a], T[16 + 3 * a],
+                      T[17 + 3 * a]};
+      RC(mf_der3d_batch(s->dc[a], a, 1, 6, x, o, s->nx, s->ny, s->nz, st));
+    }
     for (int a = 0; a < 3; ++a)
       for (int c = 0; c < 3; ++c) n.e[a][c] = T[15 + 3 * a + c];
+  } else {
+    for (int a = 0; a < 3; ++a) RC(der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
   }
+  s->grad_level = L;
   for (int c = 0; c < 3; ++c) n.out[c] = s->nl[c][L];
   n.N = N;
   n.nlt = s->nlt;

@@ -259,7 +259,7 @@ def test_ns_steps_match_oracle(tou, top):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("prec", [1, 2, 3])
 def test_ns_mixed_velocity_solves(prec):
     import mflops as mf
     mf.load()
@@ -304,10 +304,13 @@ def test_ns_les_matches_oracle(les_type, cs):
     assert np.abs(orc.les_nu).max() > 0
     # the sigma model's s3 = sqrt(i1 - s1 - s2) carries eps i1 / (2 s3^2)
     # relative error, large where the flow is nearly two-dimensional (the
-    # channel's base shear): it agrees to the conditioning of its formula
-    tol = 1e-9 if les_type < 3 else 1e-7
+    # channel's base shear): nu itself (proportional to s3) agrees to about
+    # sqrt(eps) of its largest value, the flow it drives to 1e-7
+    tol = {"u": 1e-9, "v": 1e-9, "w": 1e-9, "p": 1e-9, "nu": 1e-9}
+    if les_type == 3:
+        tol = {"u": 1e-7, "v": 1e-7, "w": 1e-7, "p": 1e-7, "nu": 1e-6}
     errs = {}
     for name, a, b in zip(("u", "v", "w", "p", "nu"), got, want):
         errs[name] = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
         print("  %s: max|gpu-oracle|/max|oracle| = %.2e" % (name, errs[name]), flush=True)
-    assert max(errs.values()) < tol, errs
+    assert all(errs[k] < tol[k] for k in errs), errs
