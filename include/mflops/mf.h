@@ -292,6 +292,44 @@ int mf_ns_set_les(mf_ns *ns, int les_type, double c, void *stream);
 /* nu of the last step with LES into a device field (<- the les_nu output
  * field, class_navier_3D.f90:101). */
 int mf_ns_les_nu(const mf_ns *ns, double *nu, void *stream);
+/* <- restart_write (class_navier_3D.f90:186-257): directory dirname (NULL:
+ *    "restart_" // tn(time), tn = floor(time) and 8 decimals, 8 digits each,
+ *    :169-183; its name into dir_out when given) holding param.dat (time as
+ *    real(8), then it(1:4) as integer(4), unformatted stream, native byte
+ *    order) and NetCDF files: for each level i of it(:) vel_u_i, vel_v_i,
+ *    vel_w_i, vel_p_i, vel_phi_i (FP64; variables U, V, W, P, P) and
+ *    rhs_u_i .. rhs_p_i, then rhs_phi_4 (RHS_*; written as zero fields: the
+ *    reference rebuilds them from zero before each use, :1615, :1810), and the
+ *    latest level in FP32 as vel_u, vel_v, vel_w, vel_p (the reference puts
+ *    w's data in vel_p, :256; here vel_p holds p).  Synchronises. */
+int mf_ns_restart_write(const mf_ns *ns, const char *dirname, char *dir_out, int dir_out_len,
+                        void *stream);
+/* <- restart_read (:259-340): time, it(:) and every level's u, v, w, p, phi
+ *    from dirname (NULL: "restart"); the phi gradients of the boundary
+ *    correction are recomputed and the flow-rate force re-initialises at the
+ *    next step (its state is not part of the files).  MF_ERR_ARG when
+ *    param.dat is missing or a file does not match the handle's sizes. */
+int mf_ns_restart_read(mf_ns *ns, const char *dirname, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Field files (NetCDF classic, host arrays; no GPU needed).
+ * --------------------------------------------------------------------- */
+/* <- write_field / write_var3d (class_field.f90:528-579, class_io.f90:23-124):
+ *    f(nx,ny,nz) (host, column-major) as variable var_name on dimensions
+ *    dim_names (NULL: "resolution_x", "resolution_y", "resolution_z", the
+ *    type(field) defaults) in FP64 (dbl != 0) or FP32.  An existing file
+ *    keeps its other variables; dimensions of the same name must have the
+ *    same length (the reference's "wrong dimensions" stop -> MF_ERR_ARG). */
+int mf_io_write_field(const char *path, const char *var_name, const char *const dim_names[3],
+                      int nx, int ny, int nz, const double *f, int dbl);
+/* <- get_var3d_info (class_io.f90:191-223): sizes (x, y, z) of var_name,
+ *    the dimension names into dim_names (3 strings of name_len bytes, may be
+ *    NULL) and whether it is stored in FP64. */
+int mf_io_field_info(const char *path, const char *var_name, int dims[3], char *dim_names,
+                     int name_len, int *is_double);
+/* <- read_var3d (class_io.f90:126-189): var_name into f(nx,ny,nz) (host,
+ *    FP64; FP32 files are converted); the sizes must match. */
+int mf_io_read_field(const char *path, const char *var_name, int nx, int ny, int nz, double *f);
 
 /* ---------------------------------------------------------------------
  * Spline filter, eddy viscosities and the learned SGS closure.

@@ -26,7 +26,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wno-unused-result",
 ]
-SOURCES = ["api.cpp", "setup.cpp", "der.hip", "solve.hip", "ns.hip", "md.hip", "les.hip"]
+SOURCES = ["api.cpp", "setup.cpp", "io.cpp", "der.hip", "solve.hip", "ns.hip", "md.hip", "les.hip"]
 
 
 def _compile(src: str) -> str:

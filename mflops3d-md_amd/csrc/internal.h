@@ -64,6 +64,13 @@ void sgs_features_launch(const double *const du[9], const double *d2x, const dou
 void sgs_aux_launch(const float *Y, int j, double tscale, double *o0, double *o1, double *o2,
                     int nx, int ny, int nz, void *stream);
 
+// NetCDF classic field files (io.cpp): one 3-D variable f(nx,ny,nz) of a
+// host array, FP64 (dbl) or FP32 on disk; reading converts to FP64.
+int nc_write_field(const char *path, const char *var_name, const char *const dim_names[3], int nx,
+                   int ny, int nz, const double *f, bool dbl, std::string &err);
+int nc_read_field(const char *path, const char *var_name, int dims[3], std::string dim_names[3],
+                  bool *is_double, double *f, std::string &err);
+
 // thread-local last error message (mf_last_error)
 int fail(int code, const std::string &msg);
 

@@ -17,7 +17,12 @@
 // velocity solve reads back its residual norms.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -810,6 +815,142 @@ extern "C" int mf_ns_set_sgs(mf_ns *s, const mf_mlp *mlp, double gscale, double 
   s->sgs_l = lscale;
   s->sgs_t = tscale;
   (void)stream;
+  return MF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Restart files (restart_write / restart_read, class_navier_3D.f90:186-340)
+// ---------------------------------------------------------------------------
+namespace {
+const char *const kDims[3] = {"resolution_x", "resolution_y", "resolution_z"};
+
+// tn(nav) (:169-183): floor(time) and the first 8 decimals, 8 digits each
+std::string time_name(double t) {
+  char b[64];
+  const double f = std::floor(t);
+  std::snprintf(b, sizeof(b), "%08lld.%08lld", (long long)f, (long long)((t - f) * 1e8));
+  return b;
+}
+}  // namespace
+
+extern "C" int mf_ns_restart_write(const mf_ns *s, const char *dirname, char *dir_out,
+                                   int dir_out_len, void *stream) {
+  if (!s) return mf::fail(MF_ERR_ARG, "mf_ns_restart_write: null handle");
+  const std::string dir = (dirname && *dirname) ? std::string(dirname) : "restart_" + time_name(s->time);
+  if (::mkdir(dir.c_str(), 0755) != 0 && errno != EEXIST)
+    return mf::fail(MF_ERR_ARG, "mf_ns_restart_write: cannot create " + dir);
+  NSCHK(hipStreamSynchronize((hipStream_t)stream));
+  // param.dat: time (real 8) then it(1:nt) (integer 4), unformatted stream
+  {
+    FILE *fp = std::fopen((dir + "/param.dat").c_str(), "wb");
+    if (!fp) return mf::fail(MF_ERR_ARG, "mf_ns_restart_write: cannot write " + dir + "/param.dat");
+    int32_t it1[NT];
+    for (int m = 0; m < NT; ++m) it1[m] = s->it[m] + 1;
+    bool ok = std::fwrite(&s->time, sizeof(double), 1, fp) == 1 &&
+              std::fwrite(it1, sizeof(int32_t), NT, fp) == (size_t)NT;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) return mf::fail(MF_ERR_ARG, "mf_ns_restart_write: write error on param.dat");
+  }
+  std::vector<double> h(s->N), zero(s->N, 0.0);
+  std::string err;
+  auto put = [&](const std::string &file, const char *name, const double *src, bool dbl) -> int {
+    const double *host = zero.data();
+    if (src) {
+      hipError_t e = hipMemcpy(h.data(), src, s->N * sizeof(double), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return mf::fail(MF_ERR_DEVICE, std::string("mf_ns_restart_write: ") + hipGetErrorString(e));
+      host = h.data();
+    }
+    const int rc = mf::nc_write_field((dir + "/" + file + ".nc").c_str(), name, kDims, s->nx, s->ny,
+                                      s->nz, host, dbl, err);
+    return rc == MF_OK ? MF_OK : mf::fail(rc, "mf_ns_restart_write: " + err);
+  };
+  // every time level in the order of it(:), in FP64; the right-hand sides
+  // fu, fv, fw, fp, fphi are rebuilt from zero before each use in the
+  // reference (:1615, :1810), so their files are written as zero fields
+  for (int i = 1; i <= NT; ++i) {
+    const int L = s->it[i - 1];
+    const std::string n = std::to_string(i);
+    RC(put("vel_u_" + n, "U", s->q[0][L], true));
+    RC(put("vel_v_" + n, "V", s->q[1][L], true));
+    RC(put("vel_w_" + n, "W", s->q[2][L], true));
+    RC(put("vel_p_" + n, "P", s->p[L], true));
+    RC(put("vel_phi_" + n, "P", s->phi[L], true));
+    RC(put("rhs_u_" + n, "RHS_U", nullptr, true));
+    RC(put("rhs_v_" + n, "RHS_V", nullptr, true));
+    RC(put("rhs_w_" + n, "RHS_W", nullptr, true));
+    RC(put("rhs_p_" + n, "RHS_P", nullptr, true));
+  }
+  RC(put("rhs_phi_" + std::to_string(NT), "RHS_PHI", nullptr, true));
+  // the latest level in single precision (the reference writes w's data to
+  // vel_p, :256; here vel_p holds p)
+  const int Ln = s->it[NT - 1];
+  RC(put("vel_u", "U", s->q[0][Ln], false));
+  RC(put("vel_v", "V", s->q[1][Ln], false));
+  RC(put("vel_w", "W", s->q[2][Ln], false));
+  RC(put("vel_p", "P", s->p[Ln], false));
+  if (dir_out && dir_out_len > 0) {
+    std::strncpy(dir_out, dir.c_str(), dir_out_len - 1);
+    dir_out[dir_out_len - 1] = 0;
+  }
+  return MF_OK;
+}
+
+extern "C" int mf_ns_restart_read(mf_ns *s, const char *dirname, void *stream) {
+  if (!s) return mf::fail(MF_ERR_ARG, "mf_ns_restart_read: null handle");
+  const std::string dir = (dirname && *dirname) ? std::string(dirname) : "restart";
+  hipStream_t st = (hipStream_t)stream;
+  double time = 0.0;
+  int32_t it1[NT];
+  {
+    FILE *fp = std::fopen((dir + "/param.dat").c_str(), "rb");
+    if (!fp) return mf::fail(MF_ERR_ARG, "mf_ns_restart_read: no " + dir + "/param.dat");
+    const bool ok = std::fread(&time, sizeof(double), 1, fp) == 1 &&
+                    std::fread(it1, sizeof(int32_t), NT, fp) == (size_t)NT;
+    std::fclose(fp);
+    if (!ok) return mf::fail(MF_ERR_ARG, "mf_ns_restart_read: short param.dat");
+  }
+  int seen = 0;
+  for (int m = 0; m < NT; ++m) {
+    if (it1[m] < 1 || it1[m] > NT || (seen & (1 << it1[m])))
+      return mf::fail(MF_ERR_ARG, "mf_ns_restart_read: param.dat levels are not a permutation of 1..4");
+    seen |= 1 << it1[m];
+  }
+  std::vector<double> h(s->N);
+  std::string err;
+  auto get = [&](const std::string &file, const char *name, double *dst) -> int {
+    int dims[3] = {s->nx, s->ny, s->nz};
+    const int rc = mf::nc_read_field((dir + "/" + file + ".nc").c_str(), name, dims, nullptr,
+                                     nullptr, h.data(), err);
+    if (rc != MF_OK) return mf::fail(rc, "mf_ns_restart_read: " + err);
+    hipError_t e = hipMemcpy(dst, h.data(), s->N * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return mf::fail(MF_ERR_DEVICE, std::string("mf_ns_restart_read: ") + hipGetErrorString(e));
+    return MF_OK;
+  };
+  NSCHK(hipStreamSynchronize(st));
+  for (int i = 1; i <= NT; ++i) {
+    const int L = it1[i - 1] - 1;
+    const std::string n = std::to_string(i);
+    RC(get("vel_u_" + n, "U", s->q[0][L]));
+    RC(get("vel_v_" + n, "V", s->q[1][L]));
+    RC(get("vel_w_" + n, "W", s->q[2][L]));
+    RC(get("vel_p_" + n, "P", s->p[L]));
+    RC(get("vel_phi_" + n, "P", s->phi[L]));
+  }
+  for (int m = 0; m < NT; ++m) s->it[m] = it1[m] - 1;
+  s->time = time;
+  // derived state: nonlinear terms recomputed when needed, the boundary
+  // gradient's phi derivatives now; the flow-rate force re-initialises from
+  // the restarted field at the next step (its state lives in save variables
+  // of f(nav,var), :1136-1147, not in the restart files)
+  for (int L = 0; L < NT; ++L) {
+    s->nl_ok[L] = false;
+    for (int a = 0; a < 3; ++a) RC(der(s, a, s->phi[L], s->dphi[L][a], st));
+    s->dphi_ok[L] = true;
+  }
+  s->grad_level = -1;
+  const double zero = 0.0;
+  NSCHK(hipMemcpyAsync(s->fstate + 2, &zero, sizeof(double), hipMemcpyHostToDevice, st));
+  NSCHK(hipStreamSynchronize(st));
   return MF_OK;
 }
 

@@ -434,6 +434,18 @@ class NavierStokes:
         constant c (squared inside, as navier_initialization does)."""
         check(self._lib.mf_ns_set_les(self._h, int(les_type), float(c), _stream(stream)))
 
+    def restart_write(self, dirname=None, stream=None) -> str:
+        """restart_write: the restart directory in the reference's layout; returns its name."""
+        out = C.create_string_buffer(1024)
+        check(self._lib.mf_ns_restart_write(self._h, dirname.encode() if dirname else None,
+                                            C.cast(out, C.c_void_p), 1024, _stream(stream)))
+        return out.value.decode()
+
+    def restart_read(self, dirname=None, stream=None):
+        """restart_read: every time level, time and level order from dirname."""
+        check(self._lib.mf_ns_restart_read(self._h, dirname.encode() if dirname else None,
+                                           _stream(stream)))
+
     def set_sgs(self, net, gscale: float = 1.0, lscale: float = 1.0, tscale: float = 1.0,
                 stream=None):
         """Learned closure (an SGSNet, or None to turn it off): tau =
@@ -706,3 +718,42 @@ def sgs_features(dcs, u, v, w, grids, gscale: float = 1.0, lscale: float = 1.0, 
                               C.c_void_p(X.data_ptr()), C.c_void_p(work.data_ptr()),
                               _stream(stream)))
     return X
+
+
+# ---------------------------------------------------------------------------
+# Field files (NetCDF classic)
+# ---------------------------------------------------------------------------
+
+FIELD_DIMS = ("resolution_x", "resolution_y", "resolution_z")
+
+
+def write_field(path, var_name: str, f: np.ndarray, dbl: bool = True, dim_names=FIELD_DIMS):
+    """write_field: host array f(nx,ny,nz) as var_name in a NetCDF classic file."""
+    lib = load()
+    a = np.asfortranarray(f, dtype=np.float64)
+    names = (C.c_char_p * 3)(*[d.encode() for d in dim_names])
+    check(lib.mf_io_write_field(str(path).encode(), var_name.encode(), C.cast(names, C.c_void_p),
+                                *a.shape, a.ctypes.data_as(C.c_void_p), int(bool(dbl))))
+
+
+def field_info(path, var_name: str):
+    """get_var3d_info: (sizes, dimension names, stored in FP64)."""
+    lib = load()
+    dims = (C.c_int * 3)()
+    names = C.create_string_buffer(3 * 256)
+    dbl = C.c_int()
+    check(lib.mf_io_field_info(str(path).encode(), var_name.encode(), C.cast(dims, C.c_void_p),
+                               C.cast(names, C.c_void_p), 256, C.byref(dbl)))
+    raw = names.raw
+    dn = tuple(raw[256 * a:256 * (a + 1)].split(b"\0")[0].decode() for a in range(3))
+    return tuple(dims), dn, bool(dbl.value)
+
+
+def read_field(path, var_name: str) -> np.ndarray:
+    """read_field: the variable as a Fortran-ordered FP64 host array."""
+    lib = load()
+    dims, _, _ = field_info(path, var_name)
+    out = np.zeros(dims, order="F")
+    check(lib.mf_io_read_field(str(path).encode(), var_name.encode(), *dims,
+                               out.ctypes.data_as(C.c_void_p)))
+    return out
