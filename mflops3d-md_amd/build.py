@@ -72,7 +72,8 @@ def build_fortran(verbose: bool = True):
     exe = os.path.join(out, "test_solver_3d_mf")
     cmd = [FC, "-O2", "-J", out, os.path.join(FORTRAN, "mflops.f90"),
            os.path.join(FORTRAN, "test_solver_3d_mf.f90"), "-o", exe,
-           "-L", LIB, "-lmflops", "-Wl,-rpath,$ORIGIN/../../lib", "-Wl,-rpath,/opt/rocm/lib"]
+           "-L", LIB, "-lmflops", "-L", "/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../lib", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True, cwd=out)
     if r.returncode != 0:
         raise RuntimeError("Fortran build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

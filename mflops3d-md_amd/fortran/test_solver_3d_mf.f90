@@ -5,12 +5,17 @@
 !   src/main/test_derivatives_omp.f90  sin(2 pi a x) + cos(2 pi b x) per axis,
 !                                      a = 1, b = 2, solver-type coefficients
 ! Grids use the reference's asin/cos stretching (class_mesh.f90:181-200) on
-! x,y in [-0.5,0.5], z in [-0.25,0.25].  Prints one "name value" line per
-! result for the calling test to check.
+! x,y in [-0.5,0.5], z in [-0.25,0.25].  Then a field file round trip
+! (write_var3d / get_var3d_info / read_var3d) and a laminar channel through
+! the NS step (the parabola with the flow-rate force is a discrete steady
+! state), written to a restart directory and continued from it.  Prints one
+! "name value" line per result for the calling test to check.
 program test_solver_3d_mf
   use iso_c_binding
   use mflops_solver_3d
   use mflops_derivatives
+  use mflops_io
+  use mflops_navier
   implicit none
   integer :: n, nx, ny, nz, i, j, k, bct(6), axis, order
   real(c_double), allocatable :: gx(:), gy(:), gz(:)
@@ -102,6 +107,88 @@ program test_solver_3d_mf
         call der_coeffs_destroy(dc)
      end do
   end do
+
+
+  ! ---- field file round trip (class_io.f90)
+  block
+    character(len=512) :: dn(3)
+    integer :: dl(3)
+    real(c_double), allocatable :: back(:,:,:)
+    call write_var3d('mf_io_test.nc', [character(len=512) :: 'resolution_x', 'resolution_y', &
+         'resolution_z'], [nx, ny, nz], 'U', u, dbl='y')
+    call get_var3d_info('mf_io_test.nc', 'U', dn, dl)
+    allocate(back(dl(1), dl(2), dl(3)))
+    call read_var3d('mf_io_test.nc', 'U', back)
+    print '(a,1x,es24.16)', 'io_roundtrip_maxdiff', maxval(abs(back - u))
+    if (dl(1) == nx .and. dl(2) == ny .and. dl(3) == nz .and. trim(dn(2)) == 'resolution_y') then
+       print '(a,1x,i0)', 'io_dims_ok', 1
+    else
+       print '(a,1x,i0)', 'io_dims_ok', 0
+    end if
+  end block
+
+  ! ---- laminar channel through the NS step, restart and continue
+  block
+    type(navier3d) :: nav, nav2
+    integer, parameter :: cx = 16, cy = 24, cz = 12
+    real(c_double) :: cgx(cx), cgy(cy), cgz(cz), re, dt
+    real(c_double), allocatable :: cu(:,:,:), cv(:,:,:), par(:,:,:), a1(:,:,:), a2(:,:,:)
+    real(c_double), allocatable :: bx(:,:,:), by(:,:,:), bz(:,:,:), zx(:,:,:), zy(:,:,:), zz(:,:,:)
+    re = 3000.0_c_double
+    dt = 1.0e-3_c_double
+    do i = 1, cx
+       cgx(i) = 2.0_c_double * pi * (i - 1) / (cx - 1)
+    end do
+    do j = 1, cy
+       cgy(j) = -1.0_c_double + (1.0_c_double + tanh(2.5_c_double * (2.0_c_double * (j - 1) / (cy - 1) - &
+            1.0_c_double)) / tanh(2.5_c_double))
+    end do
+    do k = 1, cz
+       cgz(k) = pi * (k - 1) / (cz - 1)
+    end do
+    allocate(cu(cx,cy,cz), cv(cx,cy,cz), par(cx,cy,cz), a1(cx,cy,cz), a2(cx,cy,cz))
+    do k = 1, cz
+       do j = 1, cy
+          par(:, j, k) = 1.0_c_double - cgy(j)**2
+       end do
+    end do
+    cu = par
+    cv = 0.0_c_double
+    allocate(bx(cy-2,cz-2,2), by(cx-2,cz-2,2), bz(cx-2,cy-2,2))
+    allocate(zx(cy-2,cz-2,2), zy(cx-2,cz-2,2), zz(cx-2,cy-2,2))
+    bx(:,:,1) = cu(1,2:cy-1,2:cz-1) ; bx(:,:,2) = cu(cx,2:cy-1,2:cz-1)
+    by(:,:,1) = cu(2:cx-1,1,2:cz-1) ; by(:,:,2) = cu(2:cx-1,cy,2:cz-1)
+    bz(:,:,1) = cu(2:cx-1,2:cy-1,1) ; bz(:,:,2) = cu(2:cx-1,2:cy-1,cz)
+    zx = 0.0_c_double ; zy = 0.0_c_double ; zz = 0.0_c_double
+    call navier_initialization(nav, cgx, cgy, cgz, re, dt)
+    call navier_set_bc(nav, 1, bx, by, bz)
+    call navier_set_bc(nav, 2, zx, zy, zz)
+    call navier_set_bc(nav, 3, zx, zy, zz)
+    call navier_set_flow_rate(nav, .true., 2.0_c_double / re)
+    call navier_set_state(nav, cu, cv, cv)
+    call navier_step(nav, 5)
+    call navier_get(nav, u=a1)
+    print '(a,1x,es24.16)', 'ns_channel_steady', &
+         maxval(abs(a1(2:cx-1,2:cy-1,2:cz-1) - par(2:cx-1,2:cy-1,2:cz-1)))
+    print '(a,1x,es24.16)', 'ns_time', navier_time(nav)
+    call restart_write(nav, 'restart_f')
+    call navier_initialization(nav2, cgx, cgy, cgz, re, dt)
+    call navier_set_bc(nav2, 1, bx, by, bz)
+    call navier_set_bc(nav2, 2, zx, zy, zz)
+    call navier_set_bc(nav2, 3, zx, zy, zz)
+    call navier_set_flow_rate(nav2, .true., 2.0_c_double / re)
+    call restart_read(nav2, 'restart_f')
+    ! the restarted run re-initialises its flow-rate force, as the reference;
+    ! so does the continued one here, to compare like with like
+    call navier_set_flow_rate(nav, .true., 2.0_c_double / re)
+    call navier_step(nav, 3)
+    call navier_step(nav2, 3)
+    call navier_get(nav, u=a1)
+    call navier_get(nav2, u=a2)
+    print '(a,1x,es24.16)', 'ns_restart_maxdiff', maxval(abs(a1 - a2))
+    call navier_finalize(nav)
+    call navier_finalize(nav2)
+  end block
 
 contains
 

@@ -3,7 +3,9 @@
 Runs mflops3d-md_amd/fortran/_build/test_solver_3d_mf, which repeats the
 reference's test_solver_3d (x^9+y^9+xy, sigma=2) and test_derivatives_omp
 (alpha=1, beta=2) through the library, and checks each printed error
-against the same quantity computed with the oracle.
+against the same quantity computed with the oracle; then the Fortran I/O
+and NS modules: a field file round trip, the laminar channel as a steady
+state of the NS step (to rounding) and a restart continued bitwise.
 """
 import os
 import subprocess
@@ -53,11 +55,12 @@ def _oracle_errors(n):
 
 
 @pytest.mark.gpu
-def test_fortran_caller():
+def test_fortran_caller(tmp_path):
     assert os.path.exists(EXE), "Fortran driver not built (python mflops3d-md_amd/build.py)"
     n = 33
     print("run fortran driver n=%d" % n, flush=True)
-    r = subprocess.run([EXE, str(n)], capture_output=True, text=True, timeout=120)
+    # the driver writes a field file and a restart directory into its cwd
+    r = subprocess.run([EXE, str(n)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     got = {}
@@ -72,6 +75,13 @@ def test_fortran_caller():
         assert key in got, key
         print("%s fortran %.16e oracle %.16e" % (key, got[key], val))
         assert abs(got[key] - val) <= 1e-11, (key, got[key], val)
+    # field file round trip, and the NS module: a discrete steady state kept
+    # to rounding, a restart continued bitwise
+    assert got["io_roundtrip_maxdiff"] == 0.0 and got["io_dims_ok"] == 1
+    assert got["ns_channel_steady"] <= 1e-11
+    assert abs(got["ns_time"] - 5e-3) <= 1e-15
+    assert got["ns_restart_maxdiff"] == 0.0
+    assert sorted(os.listdir(tmp_path / "restart_f"))[0] == "param.dat"
     assert got["solver_mixed_vs_fp64"] <= 1e-10
     assert got["solver_mixed_residual"] <= 1e-12
     assert abs(got["solver_mixed_error"] - ref["solver_fp64_error"]) <= 1e-11
