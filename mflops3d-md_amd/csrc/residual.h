@@ -55,6 +55,8 @@ struct ResArgs {
   unsigned *amax;        // NULL or: max |r'| (float bits, atomic max per block)
   unsigned *zero;        // NULL or: nzero scalars block 0 sets to zero
   int nzero;
+  const double *zq;      // residual_zs_kernel: Mz, Rz rows on planes b(k)..b(k)+4, (n3,10)
+  int cz;                // residual_zs_kernel: output planes per block
 };
 
 // RES = false: right-hand side b' = P g_f (u is not read);

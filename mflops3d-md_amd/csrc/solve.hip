@@ -30,6 +30,7 @@
 #include "influence.h"
 #include "internal.h"
 #include "residual.h"
+#include "residual_zs.h"
 #include "mflops/mf.h"
 
 namespace mf {
@@ -172,6 +173,8 @@ struct mf_solver3d {
   double *fbf = nullptr;                // face products fbx fby fbz (mixed)
   double *norms = nullptr;              // device scalars
   double *part = nullptr;               // per-block partial sums of norms
+  double *zq = nullptr;                 // Mz, Rz rows on planes b(k)..b(k)+4, e_z1, e_z2 (n3, 12)
+  int res_cz = 0;                       // output planes per z-streaming residual block; 0: plane pipeline
   double *hnorms = nullptr;             // pinned host mirror
   void *iws = nullptr;                  // unit-impulse batch workspace (grown on demand)
   size_t iws_bytes = 0;
@@ -437,11 +440,54 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     bandc.insert(bandc.end(), c.begin(), c.end());
     bandw.insert(bandw.end(), w.begin(), w.end());
   }
+  // z rows on the planes b(k) .. b(k)+4, b(k) = clamp(k-2, 0, n3-5), for the
+  // z-streaming residual (used when every nonzero of Mz and Rz falls there)
+  std::vector<double> zq;
+  bool zs_ok = n[0] >= 4 && n[1] >= 4 && n[2] >= 5;
+  if (zs_ok) {
+    const int m3 = n[2];
+    constexpr int PT = mf::ZS_PT;
+    zq.assign((size_t)PT * m3, 0.0);
+    for (int k = 0; k < m3; ++k) {  // e_z1, e_z2
+      zq[(size_t)PT * k + 2 * mf::BAND_W] = ebd[2 * (size_t)(n[0] + n[1]) + k];
+      zq[(size_t)PT * k + 2 * mf::BAND_W + 1] = ebd[2 * (size_t)(n[0] + n[1]) + m3 + k];
+    }
+    const size_t cb = 2 * (size_t)(n[0] + n[1]), wb = 8 * cb;  // axis 2 in bandc / bandw
+    for (int q = 0; q < 2 && zs_ok; ++q)
+      for (int k = 0; k < m3 && zs_ok; ++k) {
+        const int b = std::min(std::max(k - 2, 0), m3 - 5);
+        const int c0 = bandc[cb + (size_t)q * m3 + k];
+        for (int m = 0; m < mf::BAND_W; ++m) {
+          const double v = bandw[wb + 8 * ((size_t)q * m3 + k) + m];
+          const int d = c0 + m - b;
+          if (d >= 0 && d < mf::BAND_W) zq[(size_t)PT * k + mf::BAND_W * q + d] = v;
+          else if (v != 0.0) zs_ok = false;
+        }
+      }
+  }
+  size_t nparts_zs = 0;
+  if (zs_ok) {
+    // about 2.5 rounds of resident blocks (two per CU) over the (i,j) tiles
+    // (fewer planes per block: more warm-up reads; more: fewer blocks in
+    // flight); MF_RESIDUAL_CZ sets the planes per block (0: plane pipeline)
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    const int tiles = ((n[0] + mf::ZS_TX - 1) / mf::ZS_TX) * ((n[1] + mf::ZS_TY - 1) / mf::ZS_TY);
+    const int nch = std::max(1, (int)std::lround(5.0 * cus / tiles));
+    int cz = (n[2] + nch - 1) / nch;
+    if (const char *env = std::getenv("MF_RESIDUAL_CZ")) cz = std::atoi(env);
+
+    h->res_cz = cz <= 0 ? 0 : std::min(std::max(cz, mf::ZS_CZMIN), mf::ZS_CZMAX);
+    if (h->res_cz > 0) nparts_zs = (size_t)tiles * ((n[2] + h->res_cz - 1) / h->res_cz);
+  }
   // device layout: one allocation carved into 256-byte aligned arrays; the
   // carve runs twice, first to size the allocation, then to assign pointers
   const size_t N = (size_t)n[0] * n[1] * n[2];
-  const size_t nparts = (size_t)((n[0] + mf::RZ_TX - 1) / mf::RZ_TX) *
-                        ((n[1] + mf::RZ_TY - 1) / mf::RZ_TY) * ((n[2] + mf::RZ_CZ - 1) / mf::RZ_CZ);
+  const size_t nparts = std::max(nparts_zs, (size_t)((n[0] + mf::RZ_TX - 1) / mf::RZ_TX) *
+                                                ((n[1] + mf::RZ_TY - 1) / mf::RZ_TY) *
+                                                ((n[2] + mf::RZ_CZ - 1) / mf::RZ_CZ));
   double *dW[3], *dV[3];
   float *fW[3], *fV[3];
   auto carve = [&](double *base) -> size_t {
@@ -467,6 +513,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->F2 = (float *)take((N + 1) / 2);
     h->R32 = (float *)take((N + 1) / 2);
     h->ebd = take(ebd.size());
+    h->zq = take(std::max<size_t>(zq.size(), 1));
     h->fbf = take(2 * ((size_t)n[1] * n[2] + (size_t)n[0] * n[2] + (size_t)n[0] * n[1]));
     h->bandc = (int *)take((bandc.size() + 1) / 2);
     h->bctd = (int *)take(4);
@@ -555,6 +602,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
   if (e == hipSuccess) e = up(h->neum, neum.data(), neum.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->bandw, bandw.data(), bandw.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->bandc, bandc.data(), bandc.size() * sizeof(int));
+  if (e == hipSuccess && !zq.empty()) e = up(h->zq, zq.data(), zq.size() * sizeof(double));
   if (e == hipSuccess) e = up(h->bctd, h->bct, 6 * sizeof(int));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&h->hnorms, (MF_MAX_REFINE + 1) * sizeof(double), hipHostMallocDefault);
@@ -864,15 +912,21 @@ mf::ResArgs res_args(mf_solver3d *h, const double *u, const double *g, double si
   a.fbz = a.fby + 2L * n1 * n3;
   a.out = h->R32;
   a.part = h->part;
+  a.zq = h->zq;
+  a.cz = h->res_cz;
   return a;
 }
 
 template <bool RES>
 int launch_residual(mf_solver3d *h, const mf::ResArgs &a, double *norm2_dev, hipStream_t st) {
-  const unsigned nb = (unsigned)(((h->n1 + mf::RZ_TX - 1) / mf::RZ_TX) *
-                                 ((h->n2 + mf::RZ_TY - 1) / mf::RZ_TY) *
-                                 ((h->n3 + mf::RZ_CZ - 1) / mf::RZ_CZ));
-  hipLaunchKernelGGL(mf::residual_kernel<RES>, dim3(nb), dim3(64, mf::RZ_NW), 0, st, a);
+  const bool zs = h->res_cz > 0;
+  const int cz = zs ? h->res_cz : mf::RZ_CZ, ty = zs ? mf::ZS_TY : mf::RZ_TY;
+  const unsigned nb = (unsigned)(((h->n1 + mf::RZ_TX - 1) / mf::RZ_TX) * ((h->n2 + ty - 1) / ty) *
+                                 ((h->n3 + cz - 1) / cz));
+  if (zs)
+    hipLaunchKernelGGL(mf::residual_zs_kernel<RES>, dim3(nb), dim3(64, mf::ZS_NW), 0, st, a);
+  else
+    hipLaunchKernelGGL(mf::residual_kernel<RES>, dim3(nb), dim3(64, mf::RZ_NW), 0, st, a);
   hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
                      (long)nb, norm2_dev);
   HIPCHK(hipGetLastError());

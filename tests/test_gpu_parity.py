@@ -261,6 +261,25 @@ def test_solve_mixed(shape, bct, sigma, inner):
         assert np.all(got[0, 0, :] == 7.0) and np.all(got[:, -1, -1] == 7.0)
 
 
+@pytest.mark.parametrize("cz", ["0", "8", "13", "64"])
+def test_solve_mixed_residual_kernels(monkeypatch, cz):
+    """Both FP64 residual kernels hold the refinement contract: the plane
+    pipeline (MF_RESIDUAL_CZ=0) and the z-streaming kernel with 8 and 13
+    output planes per block (several z chunks, a ragged last one) and with
+    one chunk.  Checked by the 80-bit residual of the returned solution."""
+    monkeypatch.setenv("MF_RESIDUAL_CZ", cz)
+    mf, torch = _mf()
+    shape, bct, sigma = (70, 29, 40), (1, 1, 2, 2, 1, 1), -300.0
+    grids, sc, g, bcs, uin = _make_problem(shape, bct, sigma, 13)
+    for inner in ("fp16x2", "fp32"):
+        print("run residual kernel cz=%s inner=%s shape=%s" % (cz, inner, shape), flush=True)
+        got, rep = _solve_gpu(mf, grids, bct, g, bcs, sigma, MIXED[inner], tol=1e-13)
+        true_res = _true_residual(sc, got, g, bcs, sigma)
+        print("  %d inner solves, residual %.2e, 80-bit residual %.2e"
+              % (rep.iterations, rep.residual, true_res), flush=True)
+        assert rep.residual <= 1e-13 and true_res <= 2e-13
+
+
 def test_solve_c1_helmholtz_and_poisson():
     """Config 1: sigma=1e3 Helmholtz and sigma=0 Poisson, RHS = operator(u)."""
     mf, torch = _mf()
