@@ -233,10 +233,13 @@ int mf_solver3d_profile_read(mf_solver3d *sc, double ms[MF_PROF_CLASSES],
  * reference): the first five transforms of a solve at `precision` (x, y, z
  * with the divide, z and y backward) launched reps times back to back on the
  * handle's own work arrays (their contents are overwritten; no caller data
- * is touched), between two events.  ms_per_launch = elapsed / (5 reps);
- * flops_per_launch (may be NULL) = the algorithmic 2 M N K averaged over the
- * five.  Per-launch event pairs (mf_solver3d_profile) also count the host's
- * enqueue gaps; this does not.  Synchronises the stream. */
+ * is touched), between two events.  They are five launches, or four for
+ * MF_MIXED_FP16X2 when its z pair runs as one kernel (n3 in 129..256); each
+ * launch reads one FP32 field and writes one.  ms_per_launch = elapsed /
+ * (launches reps); flops_per_launch (may be NULL) = the algorithmic 2 M N K
+ * of the five transforms over the launches.  Per-launch event pairs
+ * (mf_solver3d_profile) also count the host's enqueue gaps; this does not.
+ * Synchronises the stream. */
 int mf_solver3d_time_transforms(mf_solver3d *sc, int precision, int reps, double *ms_per_launch,
                                 double *flops_per_launch, void *stream);
 

@@ -27,6 +27,7 @@
 #include "gemm.h"
 #include "gemm_bf3.h"
 #include "gemm_h2.h"
+#include "gemm_h2z.h"
 #include "influence.h"
 #include "internal.h"
 #include "residual.h"
@@ -797,10 +798,15 @@ void launch_h2(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W
 #undef MF_H2
 }
 
+// The z-forward transform, the divide and the z-backward transform run as
+// one kernel (gemm_h2z_kernel) when one block holds every z of both products.
+bool h2_zfused(const mf_solver3d *h) { return h->kp[2] == 256 && h->wrows[2] == mf::H2Z_BN; }
+
 // The six transforms of transforms_bf3 on the two-part fp16 matrices.  Field
 // scales: slot 0 holds max |r'| (written by the residual kernel), slot q the
-// largest output of transform q; the last transform clears slot 0 for the
-// residual that follows (the residual clears slots 1-5).
+// largest output of transform q (the fused z pair writes slot 4); the last
+// transform clears slot 0 for the residual that follows (the residual clears
+// slots 1-5).
 void transforms_h2(mf_solver3d *h, const float *in, float *w1, float *w2, double sigma,
                    double *u, bool accumulate, hipStream_t st, bool last = true) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
@@ -821,6 +827,22 @@ void transforms_h2(mf_solver3d *h, const float *in, float *w1, float *w2, double
   using mf::EPI_STORE;
   launch_h2<float, float, true, EPI_STORE>(h, in, n1, 0, h->Whs[0], h->wsc[0][0], kx, rx, w1, n1, 0, n1, n2 * n3, n1, 1, ep, 0, 1, false, st);
   launch_h2<float, float, false, EPI_STORE>(h, w1, n1, n12, h->Whs[1], h->wsc[1][0], ky, ry, w2, n1, n12, n1, n2, n2, n3, ep, 1, 2, false, st);
+  if (h2_zfused(h)) {
+    {
+      ProfScope ps(h, P_T32, 4.0 * n12 * n3 * n3, st);
+      hipLaunchKernelGGL(mf::gemm_h2z_kernel, dim3((unsigned)((n12 + mf::H2Z_BM - 1) / mf::H2Z_BM)),
+                         dim3(256), 0, st, (const float *)w2, n12, (const _Float16 *)h->Whs[2],
+                         (const _Float16 *)h->Vhs[2], kz, rz, w1, n12, (int)n12, n3, ep,
+                         (const unsigned *)(h->fmx + 2), h->wsc[2][0], h->wsc[2][1], h->fmx + 4);
+    }
+    launch_h2<float, float, false, EPI_STORE>(h, w1, n1, n12, h->Vhs[1], h->wsc[1][1], ky, ry, w2, n1, n12, n1, n2, n2, n3, ep, 4, 5, false, st);
+    if (!last) return;
+    if (accumulate)
+      launch_h2<float, double, true, mf::EPI_SCATTER_ADD>(h, w2, n1, 0, h->Vhs[0], h->wsc[0][1], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, 5, -1, true, st);
+    else
+      launch_h2<float, double, true, mf::EPI_SCATTER>(h, w2, n1, 0, h->Vhs[0], h->wsc[0][1], kx, rx, (double *)nullptr, 0, 0, n1, n2 * n3, n1, 1, ep, 5, -1, true, st);
+    return;
+  }
   launch_h2<float, float, false, mf::EPI_DIVIDE>(h, w2, n12, 0, h->Whs[2], h->wsc[2][0], kz, rz, w1, n12, 0, (int)n12, n3, n3, 1, ep, 2, 3, false, st);
   launch_h2<float, float, false, EPI_STORE>(h, w1, n12, 0, h->Vhs[2], h->wsc[2][1], kz, rz, w2, n12, 0, (int)n12, n3, n3, 1, ep, 3, 4, false, st);
   launch_h2<float, float, false, EPI_STORE>(h, w2, n1, n12, h->Vhs[1], h->wsc[1][1], ky, ry, w1, n1, n12, n1, n2, n2, n3, ep, 4, 5, false, st);
@@ -873,6 +895,8 @@ extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int re
     else
       transforms_h2(h, h->R32, h->F1, h->F2, 0.0, nullptr, false, st, false);
   };
+  // launches per pass: five, or four when the fp16 z pair is one kernel
+  const int launches = (precision == MF_MIXED_FP16X2 && h2_zfused(h)) ? 4 : 5;
   five();  // warm
   hipError_t e = hipEventRecord(e0, st);
   for (int r = 0; r < reps && e == hipSuccess; ++r) five();
@@ -885,10 +909,10 @@ extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int re
   h->prof = prof;
   if (e != hipSuccess)
     return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_time_transforms: ") + hipGetErrorString(e));
-  *ms_per_launch = ms / (5.0 * reps);
+  *ms_per_launch = ms / ((double)launches * reps);
   if (flops_per_launch)  // 2 M N K per transform, K = n1, n2, n3, n3, n2 in turn
     *flops_per_launch = 2.0 * h->n1 * h->n2 * h->n3 *
-                        (h->n1 + 2.0 * h->n2 + 2.0 * h->n3) / 5.0;
+                        (h->n1 + 2.0 * h->n2 + 2.0 * h->n3) / launches;
   return MF_OK;
 }
 
