@@ -55,8 +55,10 @@ struct ResArgs {
   unsigned *amax;        // NULL or: max |r'| (float bits, atomic max per block)
   unsigned *zero;        // NULL or: nzero scalars block 0 sets to zero
   int nzero;
-  const double *zq;      // residual_zs_kernel: Mz, Rz rows on planes b(k)..b(k)+4, (n3,10)
+  const double *zq;      // residual_zs_kernel: Mz, Rz rows on planes b(k)..b(k)+4, e_z (n3,12)
   int cz;                // residual_zs_kernel: output planes per block
+  double *out64;         // residual_zs_kernel<false>: NULL or b' in FP64 (interior-contiguous)
+  const double *bp;      // residual_zs_kernel<true>: b' in FP64, read instead of g
 };
 
 // RES = false: right-hand side b' = P g_f (u is not read);

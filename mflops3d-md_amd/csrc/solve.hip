@@ -468,9 +468,11 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
   }
   size_t nparts_zs = 0;
   if (zs_ok) {
-    // about 2.5 rounds of resident blocks (two per CU) over the (i,j) tiles
+    // about 5 blocks per CU over the (i,j) tiles, 32 planes per block at 256^3
     // (fewer planes per block: more warm-up reads; more: fewer blocks in
-    // flight); MF_RESIDUAL_CZ sets the planes per block (0: plane pipeline)
+    // flight; measured at 256^3 with three resident blocks per CU: 16 and 32
+    // equal, 48 is 7% slower); MF_RESIDUAL_CZ sets the planes per block (0:
+    // plane pipeline)
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
@@ -958,7 +960,8 @@ int launch_residual(mf_solver3d *h, const mf::ResArgs &a, double *norm2_dev, hip
 }
 
 // r' = P g_f - B u (FP32 copy -> R32) and ||r'||^2 -> norm2_dev, in one pass
-// over u and g; the face products must be current (rhs_prime).
+// over u and g (plane pipeline; the face products must be current) or over u
+// and b' = P g_f in FP64 (z-streaming kernel; b' from this solve's rhs_prime).
 // scales: max |r'| into slot 0 of the fp16 path (cleared by the transforms
 // before), slots 1-5 cleared for the transforms after
 void with_scales(mf_solver3d *h, mf::ResArgs &a) {
@@ -973,6 +976,7 @@ int residual(mf_solver3d *h, const double *u, const double *g, double sigma, dou
   ProfScope ps(h, P_RES, 20.0 * N, st);  // algorithmic: read u and g, write r' (FP32)
   mf::ResArgs a = res_args(h, u, g, sigma);
   if (scales) with_scales(h, a);
+  if (h->res_cz > 0) a.bp = h->T1;  // b' of this solve (rhs_prime), in FP64
   return launch_residual<true>(h, a, norm2_dev, st);
 }
 
@@ -995,6 +999,7 @@ int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *
   hipLaunchKernelGGL(mf::face_fold_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, f);
   mf::ResArgs a = res_args(h, nullptr, g, 0.0);
   if (scales) with_scales(h, a);
+  if (h->res_cz > 0) a.out64 = h->T1;  // b' in FP64 for the residuals of this solve
   return launch_residual<false>(h, a, norm2_dev, st);
 }
 }  // namespace
