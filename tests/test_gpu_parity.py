@@ -261,15 +261,18 @@ def test_solve_mixed(shape, bct, sigma, inner):
         assert np.all(got[0, 0, :] == 7.0) and np.all(got[:, -1, -1] == 7.0)
 
 
+@pytest.mark.parametrize("shape", [(70, 29, 40), (180, 14, 12)])
 @pytest.mark.parametrize("cz", ["0", "8", "13", "64"])
-def test_solve_mixed_residual_kernels(monkeypatch, cz):
+def test_solve_mixed_residual_kernels(monkeypatch, cz, shape):
     """Both FP64 residual kernels hold the refinement contract: the plane
     pipeline (MF_RESIDUAL_CZ=0) and the z-streaming kernel with 8 and 13
     output planes per block (several z chunks, a ragged last one) and with
-    one chunk.  Checked by the 80-bit residual of the returned solution."""
+    one chunk; 178 interior columns give x-tiles away from both x faces (the
+    kernel's lane-shift x-stencils), 68 only tiles at a face.  Checked by
+    the 80-bit residual of the returned solution."""
     monkeypatch.setenv("MF_RESIDUAL_CZ", cz)
     mf, torch = _mf()
-    shape, bct, sigma = (70, 29, 40), (1, 1, 2, 2, 1, 1), -300.0
+    bct, sigma = (1, 1, 2, 2, 1, 1), -300.0
     grids, sc, g, bcs, uin = _make_problem(shape, bct, sigma, 13)
     for inner in ("fp16x2", "fp32"):
         print("run residual kernel cz=%s inner=%s shape=%s" % (cz, inner, shape), flush=True)
