@@ -986,7 +986,8 @@ int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *
               const double *bcz, double *norm2_dev, bool scales, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   const long N = (long)n1 * n2 * n3;
-  ProfScope ps(h, P_MULT, 12.0 * N, st);  // algorithmic: read g, write b' (FP32)
+  // algorithmic: read g, write b' (FP32, and FP64 for the z-streaming residual)
+  ProfScope ps(h, P_MULT, (h->res_cz > 0 ? 20.0 : 12.0) * N, st);
   mf::FaceFoldArgs f;
   f.n1 = n1; f.n2 = n2; f.n3 = n3;
   f.Mx = rows(h->M[0]); f.My = rows(h->M[1]); f.Mz = rows(h->M[2]);
