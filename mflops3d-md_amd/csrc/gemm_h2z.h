@@ -24,7 +24,8 @@ namespace mf {
 constexpr int H2Z_BM = 64, H2Z_BN = 256;  // rows per block; outputs per block (all of z)
 
 // W1split, W2split: fp16 [2][KP/8][WROWS][8] (scaled by wscale1, wscale2),
-// WROWS == H2Z_BN, KP == 256.  grid: ceil(M / 64) blocks of 256 threads.
+// WROWS == H2Z_BN, KP a multiple of 64 <= 256 (N <= KP).  grid: ceil(M / 64)
+// blocks of 256 threads.
 // fmax_in: the field's largest magnitude (float bits); fmax_out: receives
 // the largest stored magnitude.
 __global__ void __launch_bounds__(256, 2)

@@ -802,7 +802,7 @@ void launch_h2(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W
 
 // The z-forward transform, the divide and the z-backward transform run as
 // one kernel (gemm_h2z_kernel) when one block holds every z of both products.
-bool h2_zfused(const mf_solver3d *h) { return h->kp[2] == 256 && h->wrows[2] == mf::H2Z_BN; }
+bool h2_zfused(const mf_solver3d *h) { return h->kp[2] <= 256 && h->wrows[2] == mf::H2Z_BN; }
 
 // The six transforms of transforms_bf3 on the two-part fp16 matrices.  Field
 // scales: slot 0 holds max |r'| (written by the residual kernel), slot q the

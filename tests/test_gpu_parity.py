@@ -183,6 +183,8 @@ CASES = [
     ((34, 29, 40), (1,) * 6, 1e3),
     ((34, 29, 40), (1, 1, 2, 2, 1, 1), -300.0),
     ((66, 70, 21), (1,) * 6, 0.0),
+    # 148 interior z points: the fp16x2 solve runs its z pair as one kernel
+    ((12, 14, 150), (1, 2, 1, 1, 2, 1), 50.0),
 ]
 
 
