@@ -83,8 +83,11 @@ __global__ void __launch_bounds__(64 * ZS_NW, 6) residual_zs_kernel(ResArgs a) {
   __shared__ double WX[2 * BAND_W][SW];      // Mx, Rx rows of each staged column
   __shared__ double WY[ZS_TY][2 * BAND_W];   // My, Ry rows of the tile's rows
   __shared__ double PT[ZS_CZMAX][ZS_PT];     // the chunk's plane table
-  __shared__ double EX[RES ? 1 : 2][SW];     // e_x1, e_x2 of each staged column
-  __shared__ double FX[RES ? 1 : ZS_TY][RES ? 1 : ZS_CZMAX][2];  // x-face products (x-edge tiles)
+  // boundary-fold tables, right-hand side form only (one element in the
+  // residual form, whose b' holds the fold: three blocks per CU fit the LDS)
+  constexpr int EXN = RES ? 1 : 2, FXR = RES ? 1 : ZS_TY, FXK = RES ? 1 : ZS_CZMAX;
+  __shared__ double EX[EXN][SW];             // e_x1, e_x2 of each staged column
+  __shared__ double FX[FXR][FXK][2];         // x-face products of the tile's rows (x-edge tiles)
   const int lane = threadIdx.x, tid = lane + 64 * threadIdx.y;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);  // wave index = output row (uniform)
   const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
@@ -113,9 +116,9 @@ __global__ void __launch_bounds__(64 * ZS_NW, 6) residual_zs_kernel(ResArgs a) {
       WX[m][lane] = m < BAND_W ? a.Mx.w[8 * ic + m] : (RES ? a.Rx.w[8 * ic + m - BAND_W] : 0.0);
     }
   } else if (w == BAND_W) {
-    if (!RES) {
+    if constexpr (!RES) {
       EX[0][lane] = a.ex[ic];
-      EX[RES ? 0 : 1][lane] = a.ex[n1 + ic];
+      EX[1][lane] = a.ex[n1 + ic];
     }
   } else if (w == BAND_W + 1 && lane < ZS_TY * 2 * BAND_W / 2) {
 #pragma unroll
@@ -131,12 +134,14 @@ __global__ void __launch_bounds__(64 * ZS_NW, 6) residual_zs_kernel(ResArgs a) {
     PT[kk][c] = (RES || c < BAND_W || c >= 2 * BAND_W) ? v : 0.0;
   }
   const bool xedge = !RES && (i0 < 2 || i0 + ZS_TX > n1 - 2);  // columns with e_x != 0
-  if (xedge)
-    for (int e = tid; e < ZS_TY * nz * 2; e += NT) {
-      const int r = e / (2 * nz), kk = (e / 2) % nz, side = e % 2;
-      const int jj = min(j0 + r, n2 - 1);
-      FX[RES ? 0 : r][RES ? 0 : kk][side] = a.fbx[side * sx + jj + (long)n2 * (z0 + kk)];
-    }
+  if constexpr (!RES) {
+    if (xedge)
+      for (int e = tid; e < ZS_TY * nz * 2; e += NT) {
+        const int r = e / (2 * nz), kk = (e / 2) % nz, side = e % 2;
+        const int jj = min(j0 + r, n2 - 1);
+        FX[r][kk][side] = a.fbx[side * sx + jj + (long)n2 * (z0 + kk)];
+      }
+  }
   // ---- the wave's row: window starts (staged rows), boundary fold
   const int j = j0 + w, jc = min(j, n2 - 1);
   const int cym = a.My.c0[jc] - (j0 - RZ_H);
@@ -287,10 +292,12 @@ __global__ void __launch_bounds__(64 * ZS_NW, 6) residual_zs_kernel(ResArgs a) {
       if (RES) {
         v = xk - v;
       } else {
-        if (xedge) {
-          const double ex1 = EX[0][lane], ex2 = EX[RES ? 0 : 1][lane];
-          if (ex1 != 0.0) v = fma(-ex1, FX[RES ? 0 : w][RES ? 0 : kk][0], v);
-          if (ex2 != 0.0) v = fma(-ex2, FX[RES ? 0 : w][RES ? 0 : kk][1], v);
+        if constexpr (!RES) {
+          if (xedge) {
+            const double ex1 = EX[0][lane], ex2 = EX[1][lane];
+            if (ex1 != 0.0) v = fma(-ex1, FX[w][kk][0], v);
+            if (ex2 != 0.0) v = fma(-ex2, FX[w][kk][1], v);
+          }
         }
         if (ybnd) v = fma(-eyv, xk, v);
         const double ez1 = PT[kk][2 * BAND_W], ez2 = PT[kk][2 * BAND_W + 1];
