@@ -1063,6 +1063,10 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   }
   double rel = 0.0;
   for (;;) {
+    // u's faces from this sweep's interior, enqueued before the host reads the
+    // norms (no launch after the wait); a further sweep writes them again
+    rc = launch_faces(h, u, bcx, bcy, bcz, st);
+    if (rc != MF_OK) return rc;
     HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, (it + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const double b2 = h->hnorms[0];
@@ -1079,8 +1083,6 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
     rc = residual(h, u, g, sigma, h->norms + it, scales, st);
     if (rc != MF_OK) return rc;
   }
-  rc = launch_faces(h, u, bcx, bcy, bcz, st);
-  if (rc != MF_OK) return rc;
   if (!(rel <= tol))
     return mf::fail(MF_ERR_NOCONV, "mf_solver3d_solve: refinement stopped at relative residual " +
                                        std::to_string(rel));
