@@ -132,4 +132,60 @@ __global__ void __launch_bounds__(64) face_der_kernel(const double *__restrict__
   o[foff + fsz + idx] = -dn;
 }
 
+// x-lines (contiguous): one wavefront per line, lanes along the line, so the
+// loads coalesce (face_der_kernel<0> gives each thread a whole line, and a
+// wave's threads read 64 lines a row apart); the two end-row dot products
+// are reduced across the wave.  grid: (ceil(n2/4), n3, B), block (64, 4).
+__global__ void __launch_bounds__(256) face_der_x_kernel(const double *__restrict__ U,
+                                                         double *__restrict__ out, FaceDerArgs a) {
+  const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+  const int lane = threadIdx.x;
+  const int t1 = blockIdx.x * 4 + threadIdx.y;  // j
+  const int t2 = blockIdx.y;                     // k
+  const int b = blockIdx.z;
+  if (t1 >= n2) return;  // wave-uniform
+  const int n = n1;
+  const long N = (long)n1 * n2 * n3;
+  const double *u = U + (long)b * N + (long)n1 * (t1 + (long)n2 * t2);
+  const long t = a.first + b;
+  const int sa = a.face / 2, side = a.face % 2;
+  const bool here = (sa == 0) && (t % n2 == t1) && (t / n2 == t2);
+  const double *nm = a.neum;
+  double bv0, bv1;
+  if (a.bct[0] == 1) {
+    bv0 = (here && side == 0) ? 1.0 : 0.0;
+  } else {
+    double acc = 0.0;
+    for (int m = 1; m < 7; ++m) acc -= nm[2 * m] * u[m - 1];
+    bv0 = acc * (1.0 / nm[0]);
+  }
+  if (a.bct[1] == 1) {
+    bv1 = (here && side == 1) ? 1.0 : 0.0;
+  } else {
+    double acc = 0.0;
+    for (int m = 5; m >= 0; --m) acc -= nm[1 + 2 * m] * u[n - 6 + m];
+    bv1 = acc * (1.0 / nm[13]);
+  }
+  const double *r0 = a.drow, *rn = r0 + (n + 2);
+  double d0 = 0.0, dn = 0.0;
+  for (int q = lane; q < n; q += 64) {
+    const double v = u[q];
+    d0 = fma(r0[q + 1], v, d0);
+    dn = fma(rn[q + 1], v, dn);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    d0 += __shfl_xor(d0, off, 64);
+    dn += __shfl_xor(dn, off, 64);
+  }
+  if (lane == 0) {
+    d0 = fma(r0[0], bv0, fma(r0[n + 1], bv1, d0));
+    dn = fma(rn[0], bv0, fma(rn[n + 1], bv1, dn));
+    double *o = out + (long)b * a.out_stride;
+    const long fsz = (long)n2 * n3, idx = t1 + (long)n2 * t2;
+    o[idx] = d0;
+    o[fsz + idx] = -dn;
+  }
+}
+
 }  // namespace mf

@@ -1228,7 +1228,7 @@ extern "C" int mf_influence_impulses(mf_solver3d *h, const mf_der *dx, const mf_
     double *o = out + b0 * stride;
     {
       ProfScope ps(h, P_FACES, 24.0 * N * nb, st);
-      hipLaunchKernelGGL((mf::face_der_kernel<0>), dim3((n2 + 63) / 64, n3, nb), dim3(64), 0, st,
+      hipLaunchKernelGGL(mf::face_der_x_kernel, dim3((n2 + 3) / 4, n3, nb), dim3(64, 4), 0, st,
                          (const double *)W2, o, fa);
       hipLaunchKernelGGL((mf::face_der_kernel<1>), dim3((n1 + 63) / 64, n3, nb), dim3(64), 0, st,
                          (const double *)W2, o, fa);
