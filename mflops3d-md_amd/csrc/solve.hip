@@ -1212,8 +1212,8 @@ extern "C" int mf_influence_impulses(mf_solver3d *h, const mf_der *dx, const mf_
     ia.lx = h->lam; ia.ly = h->lam + n1; ia.lz = h->lam + n1 + n2; ia.sigma = sigma;
     {
       ProfScope ps(h, P_FOLD, 8.0 * N * nb, st);
-      hipLaunchKernelGGL(mf::impulse_spectral_kernel, dim3((n1 + 63) / 64, n2, n3 * nb), dim3(64),
-                         0, st, W1, ia);
+      hipLaunchKernelGGL(mf::impulse_spectral_pq_kernel,
+                         dim3((unsigned)(((long)n1 * n2 + 255) / 256), 1, nb), dim3(256), 0, st, W1, ia);
     }
     // backward transforms, batched over the impulses
     launch_gemm<double, double, double, double, false, mf::EPI_STORE>(
@@ -1230,9 +1230,11 @@ extern "C" int mf_influence_impulses(mf_solver3d *h, const mf_der *dx, const mf_
       ProfScope ps(h, P_FACES, 24.0 * N * nb, st);
       hipLaunchKernelGGL(mf::face_der_x_kernel, dim3((n2 + 3) / 4, n3, nb), dim3(64, 4), 0, st,
                          (const double *)W2, o, fa);
-      hipLaunchKernelGGL((mf::face_der_kernel<1>), dim3((n1 + 63) / 64, n3, nb), dim3(64), 0, st,
+      hipLaunchKernelGGL((mf::face_der_yz_kernel<1>),
+                         dim3((unsigned)(((long)n1 * n3 + 255) / 256), 1, nb), dim3(256), 0, st,
                          (const double *)W2, o, fa);
-      hipLaunchKernelGGL((mf::face_der_kernel<2>), dim3((n1 + 63) / 64, n2, nb), dim3(64), 0, st,
+      hipLaunchKernelGGL((mf::face_der_yz_kernel<2>),
+                         dim3((unsigned)(((long)n1 * n2 + 255) / 256), 1, nb), dim3(256), 0, st,
                          (const double *)W2, o, fa);
     }
     HIPCHK(hipGetLastError());
