@@ -60,6 +60,13 @@ def test_impulses_neumann_faces():
     _case((14, 12, 11), (1, 2, 2, 1, 1, 1), 3.0, (0, 3, 4, 5), 3, 5)
 
 
+def test_impulses_long_x_lines():
+    """x-lines longer than a wavefront (78 interior points: the x-face
+    derivative kernel reduces two 64-lane chunks per line), Neumann x faces
+    included."""
+    _case((80, 10, 10), (2, 1, 1, 2, 1, 1), -10.0, (1, 2, 5), 0, 4)
+
+
 def test_impulses_batched_chunks(monkeypatch):
     """A whole face in several batches (batch cap 7) equals the oracle."""
     monkeypatch.setenv("MFLOPS_IMPULSE_BATCH", "7")

@@ -183,8 +183,10 @@ CASES = [
     ((34, 29, 40), (1,) * 6, 1e3),
     ((34, 29, 40), (1, 1, 2, 2, 1, 1), -300.0),
     ((66, 70, 21), (1,) * 6, 0.0),
-    # 148 interior z points: the fp16x2 solve runs its z pair as one kernel
+    # 148 and 200 interior z points (k padded to 192 and 256): the fp16x2
+    # solve runs its z pair as one kernel
     ((12, 14, 150), (1, 2, 1, 1, 2, 1), 50.0),
+    ((10, 12, 202), (2, 1, 1, 2, 1, 1), -20.0),
 ]
 
 
