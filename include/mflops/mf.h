@@ -276,9 +276,9 @@ int mf_ns_set_flow_rate(mf_ns *ns, int enable, double g0, void *stream);
 /* All time levels of u, v, w := the given fields (edges zeroed); p = phi = 0. */
 int mf_ns_set_state(mf_ns *ns, const double *u, const double *v, const double *w, void *stream);
 /* nsteps time steps.  The three Helmholtz velocity solves use `precision`
- * (MF_FP64, MF_MIXED or MF_MIXED_BF16X3, refined to tol); the all-Neumann
- * pressure Poisson solve is always FP64 (its null mode makes the right-hand
- * side's incompatible part an irreducible residual). */
+ * (MF_FP64, MF_MIXED, MF_MIXED_BF16X3 or MF_MIXED_FP16X2, refined to tol);
+ * the all-Neumann pressure Poisson solve is always FP64 (its null mode makes
+ * the right-hand side's incompatible part an irreducible residual). */
 int mf_ns_step(mf_ns *ns, int nsteps, int precision, double tol, void *stream);
 /* Latest time level of u, v, w, p, phi into device arrays (any may be NULL). */
 int mf_ns_get(const mf_ns *ns, double *u, double *v, double *w, double *p, double *phi,

@@ -279,7 +279,9 @@ struct mf_md {
   // initial-guess history (md_solve with inf_sol, class_md.f90:389-400,
   // 419-424): the last three interface solutions, oldest first
   bool guess = false;
-  double *hist = nullptr;  // 3 x ninf
+  double *hist = nullptr;  // 3 x ninf slots
+  int hist_head = 0;       // slot of the oldest solution; the others follow cyclically
+  double *slot(int age) const { return hist + (long)((hist_head + age) % 3) * ninf; }
   // identity of the layout for the matrix cache
   std::vector<int> npts;
   std::vector<double> grids;
@@ -671,7 +673,7 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
   // 2. GMRES(m) on M x = rhs, from x0 = 0 or, with the guess history on,
   //    x0 = s1 - 3 s2 + 3 s3 (class_md.f90:395-397)
   if (h->guess) {
-    const double *s1 = h->hist, *s2 = s1 + n, *s3 = s2 + n;
+    const double *s1 = h->slot(0), *s2 = h->slot(1), *s3 = h->slot(2);
     MDCHK(hipMemcpyAsync(w.x, s3, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(mf::md::axpby_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st, -3.0, s2, 3.0,
                        w.x, n);
@@ -765,9 +767,10 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
   const double res = bnorm > 0.0 ? std::sqrt(rn2) / bnorm : 0.0;
   if (rep) { rep->iterations = iters; rep->residual = res; }
   if (x_iface) MDCHK(hipMemcpyAsync(x_iface, w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  if (h->guess) {  // sol_old(1) <- sol_old(2) <- sol_old(3) <- x (:419-424)
-    MDCHK(hipMemcpyAsync(h->hist, h->hist + n, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    MDCHK(hipMemcpyAsync(h->hist + 2 * n, w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (h->guess) {  // sol_old(1) <- sol_old(2) <- sol_old(3) <- x (:419-424):
+    // the oldest slot takes x and becomes the newest (no overlapping copies)
+    MDCHK(hipMemcpyAsync(h->slot(0), w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    h->hist_head = (h->hist_head + 1) % 3;
   }
   // 3. final subdomain solves with the interface values
   MDRC(solve_subs(h, g, u, bc, w.x, st));
@@ -778,6 +781,7 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
 extern "C" int mf_md_set_guess(mf_md *h, int enable) {
   if (!h) return mf::fail(MF_ERR_ARG, "mf_md_set_guess: null handle");
   h->guess = enable != 0;
+  h->hist_head = 0;
   if (h->hist) MDCHK(hipMemset(h->hist, 0, 3 * (size_t)h->ninf * sizeof(double)));
   return MF_OK;
 }

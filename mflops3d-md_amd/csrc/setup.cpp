@@ -671,6 +671,24 @@ bool real_eig(int n, const double *A, double *wr, double *V, double *max_imag, c
   return true;
 }
 
+// The null mode of an axis with Neumann conditions at both ends (the
+// constant vector) has eigenvalue zero; dgeev or the QR iteration returns it
+// as rounding noise of order eps * max|lambda| / 1e6 or so.  The solve's guard
+// (class_solver_3d.f90:207-209) drops the mode when |sigma| and the three
+// eigenvalues are all below the absolute threshold 1e-11, which such noise can
+// exceed on strongly stretched grids (1.05e-11 on a 256-point tanh 2.5 axis,
+// whose largest eigenvalue is 1.5e7): the mode is then divided by the noise
+// and the pressure's constant becomes arbitrary.  Eigenvalues below
+// 1e-13 max|lambda| are therefore set to exactly zero, so that the guard
+// acts on every grid; where the reference's threshold already catches the
+// mode the solve is unchanged.
+void snap_null_eigenvalues(int n, double *eigen) {
+  double emax = 0.0;
+  for (int i = 0; i < n; ++i) emax = std::max(emax, std::fabs(eigen[i]));
+  for (int i = 0; i < n; ++i)
+    if (std::fabs(eigen[i]) <= 1e-13 * emax) eigen[i] = 0.0;
+}
+
 // Complete 1-D solver setup (solver_coeffs_init, class_solver_3d.f90:413-466).
 bool solver_setup_1d(const double *grid, int nall, const int bct[2], int so, Solver1D &out,
                      std::string &err) {
@@ -710,6 +728,7 @@ bool solver_setup_1d(const double *grid, int nall, const int bct[2], int so, Sol
     err = why;
     return false;
   }
+  snap_null_eigenvalues(n, out.eigen.data());
   out.inv_vect.assign((size_t)n * n, 0.0);
   if (!dense_inverse(n, out.vect.data(), out.inv_vect.data())) {
     err = "eigenvector matrix is singular";

@@ -128,7 +128,10 @@ def _face_normal_derivatives(dcs, u):
 
 
 class Multidomain:
-    def __init__(self, layout: Layout, sigma: float):
+    def __init__(self, layout: Layout, sigma: float, assemble: bool = True):
+        """assemble=False skips the dense influence matrix (rhs and
+        flux_jumps do not need it; solve does), for layouts too large for a
+        dense ninf x ninf matrix."""
         self.L = layout
         self.sigma = float(sigma)
         self.sc, self.dcs = {}, {}
@@ -136,7 +139,7 @@ class Multidomain:
             gx, gy, gz = (layout.grids[ax][s[ax]] for ax in range(3))
             self.sc[s] = o.SolverCoeffs3D(gx, gy, gz, layout.sub_bct(s), layout.so)
             self.dcs[s] = tuple(o.DerCoeffs(g, solver=True, so=layout.so) for g in (gx, gy, gz))
-        self.M = self.influence_matrix()
+        self.M = self.influence_matrix() if assemble else None
 
     def influence_matrix(self):
         """md_influence_matrix_set + setvalues: dense ninf x ninf."""

@@ -1,0 +1,373 @@
+"""The benchmark configurations at their full sizes, on the GPU.
+
+C1 (64^3 derivatives and solves) runs at full size in test_gpu_parity.py.
+Here:
+
+* C2 -- 256^3 single domain, tanh stretch 2.0 in y, Dirichlet, the seeded
+  8-mode right-hand side (mflops.synthetic.helmholtz_256_problem), at the NS
+  velocity sigma (-4.5e6) and at sigma = 0 (Poisson).  FP64 against the FP64
+  oracle (solve_3d, class_solver_3d.f90:83-297) to 1e-12; every mixed form
+  refined to tol 1e-12, its residual re-taken independently by the oracle's
+  80-bit C restatement of P (g - L u) (oracle/residual.c) and its distance to
+  the FP64 oracle printed and bounded.
+* C3 -- the 4 x 4 x 2 layout of slabs drawn from {48, 64, 96} (seed 1): a
+  cubic reproduced on every subdomain; for a non-polynomial solution the
+  flux jumps across every interface (multidomain_solve,
+  class_field.f90:1295-1388) vanish against the right-hand side; unit-impulse
+  columns of a 96-point subdomain against the oracle's loop of full solves
+  (md_influence_matrix_set, class_field.f90:1396-1510).
+* C4 -- the 192 x 256 x 192 channel (tanh 2.5, Re 3000, dt 1e-3): two FP64
+  steps against the NS oracle (navier3d.f90:48-286), twenty steps with the
+  fp16x2 velocity solves against twenty FP64 steps.
+* C5 -- 256^3 -> 64^3: the three filter matrices against the oracle's
+  B-spline least squares (LSQUnivariateSpline, class_filter.f90:130-174), the
+  filtered fields and sub-filter stresses of a 256^3 channel field against
+  the oracle, and one loss and gradient of the closure network over 2^20
+  samples against the oracle's FP64 restatement.
+"""
+import numpy as np
+import pytest
+
+import mf_oracle as o
+import oracle_c as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def _mf():
+    import mflops
+    import torch
+
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    mflops.load()
+    return mflops, torch
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+# ------------------------------------------------------------------ C2 -----
+
+C2_SIGMAS = (-4.5e6, 0.0)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    from mflops import synthetic
+    mf, torch = _mf()
+    print("run c2 set-up: 256^3 problem, oracle coefficients", flush=True)
+    gx, gy, gz, g = synthetic.helmholtz_256_problem()
+    n = g.shape[0]
+    state = {
+        "g": g,
+        "sc_o": o.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6),
+        "sc": mf.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6),
+        "gd": mf.to_device(g),
+        "bc": mf.BoundaryCondition.zeros(n, n, n),
+        "zero": [np.zeros((n - 2, n - 2, 2), order="F")] * 3,
+        "ref": {},
+    }
+    yield state
+    state.clear()
+    torch.cuda.empty_cache()
+
+
+def _c2_ref(c2, sigma):
+    if sigma not in c2["ref"]:
+        u = np.zeros(c2["g"].shape, order="F")
+        o.solver_3d(c2["sc_o"], c2["g"], u, *c2["zero"], sigma)
+        c2["ref"][sigma] = u
+    return c2["ref"][sigma]
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("sigma", C2_SIGMAS)
+def test_c2_fp64_matches_oracle(c2, sigma):
+    mf, torch = _mf()
+    ref = _c2_ref(c2, sigma)
+    print("run c2 fp64 sigma=%g" % sigma, flush=True)
+    u = mf.fortran_zeros(c2["g"].shape)
+    c2["sc"].solve(c2["gd"], u, c2["bc"], sigma, mf.MF_FP64)
+    got = mf.to_host(u)
+    err = _rel(got, ref)
+    r_gpu = oc.relative_residual(c2["sc_o"], got, c2["g"], sigma, *c2["zero"])
+    r_ora = oc.relative_residual(c2["sc_o"], ref, c2["g"], sigma, *c2["zero"])
+    print("c2 fp64 sigma=%g: |gpu-oracle64| %.2e; residual (80-bit) gpu %.2e, oracle64 %.2e"
+          % (sigma, err, r_gpu, r_ora), flush=True)
+    assert err <= 1e-12
+    assert r_gpu <= 1e-12
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("prec", [1, 2, 3], ids=["fp32", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("sigma", C2_SIGMAS)
+def test_c2_mixed_residual_and_error(c2, sigma, prec):
+    mf, torch = _mf()
+    ref = _c2_ref(c2, sigma)
+    print("run c2 mixed precision=%d sigma=%g" % (prec, sigma), flush=True)
+    u = mf.fortran_zeros(c2["g"].shape)
+    rep = c2["sc"].solve(c2["gd"], u, c2["bc"], sigma, prec, tol=1e-12)
+    got = mf.to_host(u)
+    r_ind = oc.relative_residual(c2["sc_o"], got, c2["g"], sigma, *c2["zero"])
+    err = _rel(got, ref)
+    print("c2 precision=%d sigma=%g: %d inner solves, residual reported %.2e, re-taken (80-bit) "
+          "%.2e, |mixed-oracle64| %.2e" % (prec, sigma, rep.iterations, rep.residual, r_ind, err),
+          flush=True)
+    assert rep.residual <= 1e-12
+    assert r_ind <= 1e-12
+    assert err <= 2e-11
+
+
+# ------------------------------------------------------------------ C3 -----
+
+C3_SIGMA = -1000.0
+
+
+def _c3_poly(X, Y, Z):
+    return X ** 3 + 2 * Y ** 3 - Z ** 2 + X * Y * Z + 0.5 * X * Y
+
+
+def _c3_poly_lap(X, Y, Z):
+    return 6 * X + 12 * Y - 2
+
+
+def _c3_sines(X, Y, Z):
+    return np.sin(np.pi * X) * np.cos(0.5 * np.pi * Y) * np.sin(np.pi * Z + 0.3)
+
+
+def _c3_sines_lap(X, Y, Z):
+    return -(2.25 * np.pi ** 2) * _c3_sines(X, Y, Z)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    import md_oracle as md
+    from mflops import synthetic
+    mf, torch = _mf()
+    slab, subs = synthetic.subdomain_layout()
+    nd = tuple(len(v) for v in slab)
+    grids = [[None] * nd[ax] for ax in range(3)]
+    for sd in subs:
+        for ax in range(3):
+            grids[ax][sd["index"][ax]] = sd["grids"][ax]
+    print("run c3 set-up: slabs x=%s y=%s z=%s, sigma %g" % (*slab, C3_SIGMA), flush=True)
+    L = md.Layout(grids)
+    h = mf.Multidomain(grids, C3_SIGMA)
+    assert h.ninf == L.ninf and h.nsub == len(L.subs) == 32
+    print("c3: %d interface unknowns, %.1f GB of influence blocks" % (h.ninf, h.matrix_bytes / 1e9),
+          flush=True)
+    state = {"grids": grids, "slab": slab, "L": L, "h": h,
+             "mdo": md.Multidomain(L, C3_SIGMA, assemble=False)}
+    yield state
+    state.clear()
+    del h
+    torch.cuda.empty_cache()
+
+
+def _c3_solve(mf, torch, c3, fun, lap, tol):
+    import md_oracle as md
+    L = c3["L"]
+    g, bc = md.manufactured(L, fun, lap, C3_SIGMA)
+    gd = [mf.to_device(g[s]) for s in L.subs]
+    bd = [torch.tensor(np.concatenate([b.ravel(order="F") for b in bc[s]]), device="cuda")
+          for s in L.subs]
+    u, rep = c3["h"].solve(gd, bd, tol=tol)
+    return g, bc, {s: mf.to_host(u[i]) for i, s in enumerate(L.subs)}, rep
+
+
+@pytest.mark.timeout(600)
+def test_c3_cubic_reproduced_on_every_subdomain(c3):
+    mf, torch = _mf()
+    L = c3["L"]
+    print("run c3 cubic", flush=True)
+    _, _, u, rep = _c3_solve(mf, torch, c3, _c3_poly, _c3_poly_lap, 1e-12)
+    worst = 0.0
+    for s in L.subs:
+        X, Y, Z = np.meshgrid(*[np.asarray(L.grids[ax][s[ax]]) for ax in range(3)], indexing="ij")
+        ex = _c3_poly(X, Y, Z)
+        e = np.abs(u[s] - ex)[1:-1, 1:-1, 1:-1].max() / np.abs(ex).max()
+        worst = max(worst, e)
+    print("c3 cubic: GMRES %d iterations, residual %.2e, worst subdomain error %.2e"
+          % (rep.iterations, rep.residual, worst), flush=True)
+    assert rep.residual <= 1e-11
+    assert worst <= 1e-9
+
+
+@pytest.mark.timeout(600)
+def test_c3_flux_jumps_vanish(c3):
+    mf, torch = _mf()
+    print("run c3 flux jumps", flush=True)
+    g, bc, u, rep = _c3_solve(mf, torch, c3, _c3_sines, _c3_sines_lap, 1e-12)
+    mdo = c3["mdo"]
+    r = mdo.rhs(g, bc)        # the jumps of the zero-interface solves
+    j = mdo.flux_jumps(u)     # the jumps of the device solution
+    print("c3 flux jumps: GMRES %d iterations, residual %.2e; max jump %.2e against max rhs %.2e"
+          % (rep.iterations, rep.residual, np.abs(j).max(), np.abs(r).max()), flush=True)
+    assert np.abs(r).max() > 0
+    assert np.abs(j).max() <= 1e-10 * np.abs(r).max()
+
+
+@pytest.mark.timeout(600)
+def test_c3_impulse_columns_of_a_96_point_subdomain(c3):
+    mf, torch = _mf()
+    L = c3["L"]
+    # the subdomain with the most 96-point axes
+    s = max(L.subs, key=lambda t: (sum(L.shape(t)[a] == 96 for a in range(3)), L.shape(t)))
+    shape = L.shape(s)
+    assert 96 in shape
+    gs = [np.asarray(L.grids[ax][s[ax]]) for ax in range(3)]
+    bct = L.sub_bct(s)
+    face = L.faces(s)[0][0]
+    npts = mf.face_points(shape, face)
+    first, count = npts // 2 - 4, 8
+    print("run c3 impulses subdomain %s shape %s face %s first %d count %d"
+          % (s, shape, mf.FACES[face], first, count), flush=True)
+    sc = mf.SolverCoeffs3D(*gs, bct, 6)
+    dcs = [mf.DerivativeCoefficients(gg, solver=True, so=6) for gg in gs]
+    got = mf.influence_impulses(sc, dcs, C3_SIGMA, face, first, count).cpu().numpy()
+    osc = o.SolverCoeffs3D(*gs, bct, 6)
+    odc = [o.DerCoeffs(gg, solver=True, so=6) for gg in gs]
+    ref = o.influence_impulses(osc, odc, C3_SIGMA, face, first, count)
+    err = _rel(got, ref)
+    print("c3 impulses: |gpu-oracle64| %.2e of the largest response" % err, flush=True)
+    assert got.shape == ref.shape
+    assert err <= 1e-11
+
+
+# ------------------------------------------------------------------ C4 -----
+
+@pytest.fixture(scope="module")
+def c4():
+    from mflops import synthetic
+    mf, torch = _mf()
+    print("run c4 set-up: 192x256x192 channel", flush=True)
+    gx, gy, gz, u, v, w, bc = synthetic.channel_problem(192, 256, 192)
+    state = {"grids": (gx, gy, gz), "fields": (u, v, w), "bc": bc, "re": 3000.0, "dt": 1e-3}
+    yield state
+    state.clear()
+    torch.cuda.empty_cache()
+
+
+def _c4_device(mf, c4):
+    s = mf.NavierStokes(*c4["grids"], c4["re"], c4["dt"])
+    for c in range(3):
+        s.set_bc(c, mf.BoundaryCondition(*[mf.to_device(b) for b in c4["bc"][c]]))
+    s.set_state(*[mf.to_device(f) for f in c4["fields"]])
+    s.set_flow_rate(True, 2.0 / c4["re"])
+    return s
+
+
+@pytest.mark.timeout(900)
+def test_c4_fp64_steps_match_oracle(c4):
+    import ns_oracle as ns
+    mf, torch = _mf()
+    steps = 2
+    print("run c4 oracle, %d steps" % steps, flush=True)
+    orc = ns.NSOracle(*c4["grids"], c4["re"], c4["dt"])
+    orc.set_state(*c4["fields"])
+    for c in range(3):
+        orc.bc["uvw"[c]] = c4["bc"][c]
+    orc.flow = ns.FlowRate(g0=2.0 / c4["re"])
+    for _ in range(steps):
+        orc.step()
+    print("run c4 device fp64, %d steps" % steps, flush=True)
+    dev = _c4_device(mf, c4)
+    dev.step(steps, mf.MF_FP64)
+    got = [mf.to_host(t) for t in dev.get()]
+    want = list(orc.velocity()) + [orc.pressure(), orc.phi_latest()]
+    errs = {name: _rel(a, b) for name, a, b in zip(("u", "v", "w", "p", "phi"), got, want)}
+    print("c4 fp64 vs oracle after %d steps: %s" % (steps, {k: "%.2e" % v for k, v in errs.items()}),
+          flush=True)
+    assert all(e <= 1e-10 for e in errs.values()), errs
+    t, gflow = dev.info()
+    assert abs(gflow - orc.flow.g) <= 1e-10 * abs(orc.flow.g)
+
+
+@pytest.mark.timeout(600)
+def test_c4_mixed_steps_match_fp64(c4):
+    mf, torch = _mf()
+    steps = 20
+    runs = {}
+    for name, prec in (("fp64", mf.MF_FP64), ("fp16x2", mf.MF_MIXED_FP16X2)):
+        print("run c4 device %s, %d steps" % (name, steps), flush=True)
+        dev = _c4_device(mf, c4)
+        dev.step(steps, prec, 1e-12)
+        runs[name] = [mf.to_host(t) for t in dev.get()[:4]]
+        del dev
+    errs = {n: _rel(b, a) for n, a, b in zip("uvwp", runs["fp64"], runs["fp16x2"])}
+    print("c4 fp16x2 vs fp64 after %d steps: %s" % (steps, {k: "%.2e" % v for k, v in errs.items()}),
+          flush=True)
+    assert all(e <= 1e-10 for e in errs.values()), errs
+
+
+# ------------------------------------------------------------------ C5 -----
+
+@pytest.fixture(scope="module")
+def c5():
+    from mflops import synthetic
+    mf, torch = _mf()
+    print("run c5 set-up: 256^3 channel field (Re 5000), 64^3 coarse grid", flush=True)
+    gx, gy, gz, u, v, w, _ = synthetic.channel_problem(256, 256, 256, re=5000.0, amp=0.3, kmax=8)
+    cg = synthetic.channel_problem(64, 64, 64, re=5000.0, amp=0.0)[:3]
+    state = {"fine": (gx, gy, gz), "coarse": cg, "fields": (u, v, w)}
+    yield state
+    state.clear()
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.timeout(300)
+def test_c5_filter_matrices(c5):
+    import les_oracle as les
+    mf, torch = _mf()
+    print("run c5 filter matrices 256 -> 64", flush=True)
+    for a in range(3):
+        fi, co = c5["fine"][a], c5["coarse"][a]
+        F = mf.SplineFilter(fi, co, co, 3).matrix()
+        R = les.lsq_filter_matrix(fi, co, co, 3)
+        err = _rel(F, R)
+        print("c5 filter axis %d: %s, |F-oracle| %.2e" % (a, F.shape, err), flush=True)
+        assert F.shape == (64, 256)
+        assert err <= 1e-12
+
+
+@pytest.mark.timeout(600)
+def test_c5_sgs_targets_256_to_64(c5):
+    import les_oracle as les
+    mf, torch = _mf()
+    fs = [mf.SplineFilter(c5["fine"][a], c5["coarse"][a], c5["coarse"][a], 3) for a in range(3)]
+    Fs = [les.lsq_filter_matrix(c5["fine"][a], c5["coarse"][a], c5["coarse"][a], 3) for a in range(3)]
+    print("run c5 sgs targets 256^3 -> 64^3", flush=True)
+    bars, tau = mf.sgs_targets(*fs, *[mf.to_device(f) for f in c5["fields"]], tscale=0.01)
+    qb, tref = les.sgs_targets(*Fs, *c5["fields"], tscale=0.01)
+    e_bar = max(_rel(mf.to_host(a), b) for a, b in zip(bars, qb))
+    tau = tau.cpu().numpy()
+    e_tau = _rel(tau, tref)
+    print("c5 sgs targets: filtered velocity %.2e, tau (float32) %.2e" % (e_bar, e_tau), flush=True)
+    assert tau.shape == tref.shape == (64 ** 3, 6)
+    assert e_bar <= 1e-12
+    assert e_tau <= 1e-6
+
+
+@pytest.mark.timeout(300)
+def test_c5_loss_and_gradient_over_a_million_samples():
+    import les_oracle as les
+    mf, torch = _mf()
+    n = 1 << 20
+    rng = np.random.default_rng(55)
+    X = rng.uniform(-1, 1, (n, 10)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0] + X[:, 1]), X[:, 2] * X[:, 3], np.tanh(X[:, 4]),
+                  0.5 * X[:, 5], np.cos(X[:, 6]) - 0.5, X[:, 7] * X[:, 8] - X[:, 9]],
+                 1).astype(np.float32)
+    p = les.init_params(8)
+    p[les.NPARAM - 6:] = 0.05
+    print("run c5 loss/gradient n=%d" % n, flush=True)
+    net = mf.SGSNet(p)
+    loss, g = net.loss_grad(torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda"))
+    lr, gr = les.loss_and_grad(p, X, T)
+    g = g.cpu().numpy()
+    e_l, e_g = abs(loss - lr) / lr, np.abs(g - gr).max() / np.abs(gr).max()
+    print("c5 loss %.6e (oracle %.6e, rel %.2e), gradient rel %.2e" % (loss, lr, e_l, e_g),
+          flush=True)
+    assert e_l <= 1e-5
+    assert e_g <= 1e-4

@@ -202,3 +202,34 @@ def test_gpu_md_guess_extrapolation():
     assert its[3] * 4 < its[0]
     for a, b in zip(us[0], us[3]):
         assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
+
+
+@pytest.mark.gpu
+def test_gpu_md_guess_three_distinct_solutions():
+    """The history holds three different solutions: right-hand sides scaled
+    by t = 1, 4, 9 give interface solutions t x, and s1 - 3 s2 + 3 s3 =
+    (1 - 12 + 27) x = 16 x is exactly the solution for t = 16.  Any slot
+    mix-up in the shift (sol_old(1) <- sol_old(2) <- sol_old(3) <- x,
+    class_md.f90:419-424) leaves a guess off by a multiple of x, which GMRES
+    then needs its full iteration count to remove."""
+    mf, torch = _mf()
+    grids = layout_3d()
+    L = md.Layout(grids)
+    sigma = -30.0
+    g, bc = md.manufactured(L, sines, sines_lap, sigma)
+    gd = [mf.to_device(g[s]) for s in L.subs]
+    bd = [torch.tensor(_bc_vec(bc[s]), device="cuda") for s in L.subs]
+    h = mf.Multidomain(grids, sigma)
+    x1 = torch.zeros(L.ninf, dtype=torch.float64, device="cuda")
+    _, rep1 = h.solve(gd, bd, tol=1e-13, x_iface=x1)
+    h.set_guess(True)
+    its = []
+    for t in (1.0, 4.0, 9.0, 16.0):
+        xi = torch.zeros_like(x1)
+        _, rep = h.solve([a * t for a in gd], [b * t for b in bd], tol=1e-11, x_iface=xi)
+        its.append(rep.iterations)
+        err = (xi - t * x1).abs().max().item() / (t * x1.abs().max().item())
+        assert err < 1e-9, "t=%g interface error %.2e" % (t, err)
+    print("GMRES iterations: cold %d, with history %s" % (rep1.iterations, its), flush=True)
+    # the fourth guess is exact up to the GMRES error of the three before it
+    assert its[3] <= 2 and its[0] >= 4 * max(its[3], 1)

@@ -419,3 +419,24 @@ def test_poisson_all_neumann_null_mode():
     # solution equals uin up to the null vector (a constant for this operator)
     diff = u[1:-1, 1:-1, 1:-1] - uin
     assert np.abs(diff - diff.mean()).max() < 1e-8
+
+
+def test_null_eigenvalue_snapped_to_zero():
+    """The constant null vector of an axis with Neumann conditions at both
+    ends gets eigenvalue exactly 0 (the reference's absolute 1e-11 guard,
+    class_solver_3d.f90:207-209, misses dgeev's 1.05e-11 on this 256-point
+    tanh-2.5 axis); no other eigenvalue changes, Dirichlet axes are untouched."""
+    g = o.grid_tanh(256, -1.0, 1.0, 2.5)
+    for bct, nzero in (((2, 2), 1), ((1, 1), 0), ((1, 2), 0)):
+        c = o.SolverCoeffs1D(g, bct, 6)
+        lhs, rhs = o.solver_lhs_rhs(c.ddl, c.ddr)
+        raw = o.solver_matrix_init(c.ddl, c.ddr)[0]
+        assert np.sum(c.eigen == 0.0) == nzero
+        changed = c.eigen != raw
+        assert np.sum(changed) == nzero
+        if nzero:
+            # the snapped one is the null mode: its eigenvector is constant
+            i = int(np.argmin(np.abs(raw)))
+            assert c.eigen[i] == 0.0 and abs(raw[i]) > 1e-11 * 0.5
+            v = c.vect[:, i]
+            assert np.abs(v - v.mean()).max() < 1e-8 * np.abs(v).max()
