@@ -360,6 +360,18 @@ int mf_filter_matrix(const mf_filter *h, double *F);
  * nxo*nyi*nzi + nxo*nyo*nzi doubles.  Device pointers. */
 int mf_filter3d(const mf_filter *fx, const mf_filter *fy, const mf_filter *fz, const double *in,
                 double *out, double *work, void *stream);
+/* mf_filter3d with no work argument and no allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * two intermediate fields live in a library-owned device array of
+ * MF_FILTER_SCRATCH_DOUBLES doubles (48 MiB, enough for 256^3 -> 64^3), one
+ * per GPU, held until the process ends; MF_ERR_ARG when
+ * nxo*nyi*nzi + nxo*nyo*nzi exceeds it.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race.
+ * Same kernels and results as mf_filter3d; enqueued on stream. */
+#define MF_FILTER_SCRATCH_DOUBLES 6291456
+int mf_filter3d_noalloc(const mf_filter *fx, const mf_filter *fy, const mf_filter *fz,
+                        const double *in, double *out, void *stream);
 
 /* <- navier_LES (src/class_navier/class_navier_3D.f90:1225-1449), les_sigma
  *    (:1453-1497): the eddy viscosity nu(nx,ny,nz) from the nine velocity

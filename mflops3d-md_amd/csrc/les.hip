@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -599,6 +600,57 @@ extern "C" int mf_filter3d(const mf_filter *fx, const mf_filter *fy, const mf_fi
                       nzi, 1, st);
   LESCHK(hipGetLastError());
   return MF_OK;
+}
+
+// ---- mf_filter3d without allocation or workspace argument -----------------
+// The two intermediate fields of mf_filter3d live in one file-scope device
+// array (each GPU holds its own copy, for the life of the process).  Host
+// code never passes the array's name to a kernel: it is a host-side
+// stand-in there.  Its device address comes from hipGetSymbolAddress, once
+// per device, with that device current, on the first call made there.
+namespace {
+static __device__ __align__(256) double scratch[MF_FILTER_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double *scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(double **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_filter3d_noalloc: hipGetDevice: ") +
+                                       hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return mf::fail(MF_ERR_DEVICE, "mf_filter3d_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return mf::fail(MF_ERR_DEVICE, std::string("mf_filter3d_noalloc: hipGetSymbolAddress: ") +
+                                         hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = scratch_ptr[dev];
+  return MF_OK;
+}
+}  // namespace
+
+extern "C" int mf_filter3d_noalloc(const mf_filter *fx, const mf_filter *fy, const mf_filter *fz,
+                                   const double *in, double *out, void *stream) {
+  if (!fx || !fy || !fz || !in || !out)
+    return mf::fail(MF_ERR_ARG, "mf_filter3d_noalloc: null pointer");
+  const size_t need = (size_t)fx->nout * fy->nin * fz->nin + (size_t)fx->nout * fy->nout * fz->nin;
+  if (need > (size_t)MF_FILTER_SCRATCH_DOUBLES)
+    return mf::fail(MF_ERR_ARG, "mf_filter3d_noalloc: nxo*nyi*nzi + nxo*nyo*nzi exceeds "
+                                "MF_FILTER_SCRATCH_DOUBLES");
+  double *ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc != MF_OK) return rc;
+  // every element of both intermediates is written before it is read: no
+  // clearing, so the previous call's contents do not matter
+  return mf_filter3d(fx, fy, fz, in, out, ws, stream);
 }
 
 void mf::les_spacing2(const double *g, int n, double *out) {

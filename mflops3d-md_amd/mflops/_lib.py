@@ -77,6 +77,7 @@ SIGNATURES = {
     "mf_filter_destroy": (C.c_int, [_vp]),
     "mf_filter_matrix": (C.c_int, [_vp, _vp]),
     "mf_filter3d": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mf_filter3d_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "mf_les_nu": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_double,
                             _vp, _vp]),
     "mf_mlp_create": (C.c_int, [_vp, C.POINTER(_vp)]),
