@@ -17,6 +17,7 @@
 // uses W'_a = inv(V_a) inv(lhs_a) in place of inv(V_a), which absorbs P^-1.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -70,6 +71,53 @@ __global__ void __launch_bounds__(256)
   v -= fz1[k] * bcz[i + (long)n1 * j];
   v -= fz2[k] * bcz[sz + i + (long)n1 * j];
   R[p] = v;
+}
+
+// Null component of a singular problem (sigma = 0 and a zero eigenvalue on
+// every axis): c0 = (w0x (x) w0y (x) w0z) . R / (sum w0x sum w0y sum w0z), with
+// w0a the row of inv(V_a) of the zero eigenvalue, is removed from the folded
+// right-hand side before the transforms.  The constant is the operator's
+// right null vector and every other left eigenvector is orthogonal to it,
+// so the modes the divide keeps are those of the reference in exact
+// arithmetic; in floating point the incompatible part would otherwise leak
+// into them through the rounding of their eigenvectors.
+// Pass 1: one block per z-plane, part[k] = w0z(k) sum_ij w0x(i) w0y(j) R(i,j,k).
+__global__ void __launch_bounds__(256)
+    null_dot_kernel(const double *__restrict__ R, int n1, int n2, const double *__restrict__ w0,
+                    int n3, double *__restrict__ part) {
+  const int k = blockIdx.x;
+  const double *wx = w0, *wy = w0 + n1, *wz = w0 + n1 + n2;
+  const long n12 = (long)n1 * n2;
+  const double *Rk = R + n12 * k;
+  double acc = 0.0;
+  for (long p = threadIdx.x; p < n12; p += 256) {
+    const int i = (int)(p % n1), j = (int)(p / n1);
+    acc += wx[i] * wy[j] * Rk[p];
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[k] = red[0] * wz[k];
+}
+
+// Pass 2: every block sums the plane partials in the same order (so all see
+// the same c0) and subtracts c0 from its points.
+__global__ void __launch_bounds__(256)
+    null_sub_kernel(double *__restrict__ R, long N, const double *__restrict__ part, int n3,
+                    double inv_wsum) {
+  __shared__ double c0s;
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < n3; ++k) t += part[k];
+    c0s = t * inv_wsum;
+  }
+  __syncthreads();
+  const double c0 = c0s;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long)gridDim.x * 256) R[p] -= c0;
 }
 
 // Faces of u: Dirichlet copies bc; Neumann solves the one-sided 7-point
@@ -177,6 +225,12 @@ struct mf_solver3d {
   double *zq = nullptr;                 // Mz, Rz rows on planes b(k)..b(k)+4, e_z1, e_z2 (n3, 12)
   int res_cz = 0;                       // output planes per z-streaming residual block; 0: plane pipeline
   double *hnorms = nullptr;             // pinned host mirror
+  // singular problems (sigma = 0, a zero eigenvalue on every axis): the
+  // null rows w0x | w0y | w0z of inv(V), 1 / (sum w0x sum w0y sum w0z), and
+  // n3 plane partials
+  bool null_mode = false;
+  double *w0 = nullptr, *npart = nullptr;
+  double inv_wsum = 0.0;
   void *iws = nullptr;                  // unit-impulse batch workspace (grown on demand)
   size_t iws_bytes = 0;
   // kernel timing (mf_solver3d_profile): event pairs recorded on the launch
@@ -532,6 +586,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
       h->Vhs[a] = (_Float16 *)take(cnh);
     }
     h->fmx = (unsigned *)take(4);
+    h->w0 = take((size_t)n[0] + n[1] + n[2]);
+    h->npart = take(n[2]);
     return off;
   };
   const size_t nd = carve(nullptr);
@@ -597,6 +653,32 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
         }
       if (e == hipSuccess)
         e = up(which == 0 ? (void *)h->Whs[a] : (void *)h->Vhs[a], hp.data(), hp.size() * 2);
+    }
+  }
+  // null rows of a singular problem (the zero eigenvalues are exact zeros)
+  {
+    std::vector<double> w0;
+    double wsum = 1.0;
+    bool all = true;
+    for (int a = 0; a < 3 && all; ++a) {
+      const mf::Solver1D &s = h->ax[a];
+      const int m = n[a];
+      int idx = -1, cnt = 0;
+      for (int i = 0; i < m; ++i)
+        if (s.eigen[i] == 0.0) { idx = i; ++cnt; }
+      if (cnt != 1) { all = false; break; }
+      double sum = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double v = s.inv_vect[idx + (size_t)m * j];
+        w0.push_back(v);
+        sum += v;
+      }
+      wsum *= sum;
+    }
+    if (all && wsum != 0.0 && std::isfinite(wsum)) {
+      h->null_mode = true;
+      h->inv_wsum = 1.0 / wsum;
+      if (e == hipSuccess) e = up(h->w0, w0.data(), w0.size() * sizeof(double));
     }
   }
   if (e == hipSuccess) e = up(h->lam, lam.data(), lam.size() * sizeof(double));
@@ -1025,6 +1107,14 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
       ProfScope ps(h, P_FOLD, 16.0 * N, st);
       hipLaunchKernelGGL(mf::fold_kernel, dim3((h->n1 + 63) / 64, (h->n2 + 3) / 4, h->n3), dim3(64, 4), 0, st, g, h->T2, h->n1, h->n2,
                          h->n3, (const double *)h->fold, bcx, bcy, bcz);
+    }
+    if (h->null_mode && sigma == 0.0) {
+      ProfScope ps(h, P_FOLD, 16.0 * N, st);
+      hipLaunchKernelGGL(mf::null_dot_kernel, dim3(h->n3), dim3(256), 0, st, (const double *)h->T2,
+                         h->n1, h->n2, (const double *)h->w0, h->n3, h->npart);
+      const long nb = std::min<long>((N + 255) / 256, 4096);
+      hipLaunchKernelGGL(mf::null_sub_kernel, dim3((unsigned)nb), dim3(256), 0, st, h->T2, N,
+                         (const double *)h->npart, h->n3, h->inv_wsum);
     }
     transforms<double, double>(h, h->T2, h->T1, h->T2, h->Wx, h->Wyt, h->Wzt, h->Vx, h->Vyt,
                                h->Vzt, sigma, u, false, st);

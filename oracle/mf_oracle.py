@@ -738,6 +738,38 @@ def _extrap(neum_row, vals_in, bc, first: bool):
     return acc * (1.0 / neum_row[6])
 
 
+def null_rows(sc: SolverCoeffs3D):
+    """Rows of inv(V) of the zero eigenvalue of each axis, or None when some
+    axis has none (then sigma = 0 is not singular)."""
+    rows = []
+    for c in (sc.cx, sc.cy, sc.cz):
+        idx = np.flatnonzero(c.eigen == 0.0)
+        if len(idx) != 1:
+            return None
+        rows.append(c.inv_vect[idx[0]])
+    return rows
+
+
+def remove_null_component(sc: SolverCoeffs3D, r):
+    """The folded right-hand side minus c0 (the constant vector), c0 =
+    w0.r / w0.1 with w0 = w0x (x) w0y (x) w0z the left null vector.
+
+    Deliberate deviation from the reference, equal to it in exact arithmetic:
+    the constant is the right null vector of the all-singular operator and
+    every other left eigenvector is orthogonal to it, so the modes the
+    eigen-divide keeps (class_solver_3d.f90:202-225) do not change.  In
+    floating point the incompatible part c0 otherwise leaks into the other
+    modes in proportion to the rounding of their left eigenvectors, which
+    differs between eigensolvers: on the 192 x 256 x 192 channel the pressure
+    moved by 2.5e-8 of its size between two FP64 eigen-decompositions."""
+    rows = null_rows(sc)
+    if rows is None:
+        return r
+    wx, wy, wz = rows
+    c0 = np.einsum("i,j,k,ijk->", wx, wy, wz, r, optimize=True) / (wx.sum() * wy.sum() * wz.sum())
+    return np.asfortranarray(r - c0)
+
+
 def solve_3d(sc: SolverCoeffs3D, rhs, sigma: float, solf, bcx, bcy, bcz):
     """solve_3d (class_solver_3d.f90:83-299).
 
@@ -746,6 +778,8 @@ def solve_3d(sc: SolverCoeffs3D, rhs, sigma: float, solf, bcx, bcy, bcz):
     """
     n1, n2, n3 = rhs.shape
     r = fold_bc(sc, rhs, bcx, bcy, bcz)
+    if sigma == 0.0:
+        r = remove_null_component(sc, r)
     aux1 = _dgx(sc.cx.inv_vect, r)
     aux2 = _dgy(sc.ivy_t, aux1)
     aux1 = _dgz(sc.ivz_t, aux2)

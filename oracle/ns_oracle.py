@@ -151,6 +151,9 @@ class NSOracle:
         self.time = 0.0
         self.les_type, self.les_c2, self.les_nu = 0, 0.0, None
         self.sgs = None
+        # called with each predicted velocity right after its solve (tests
+        # use it to perturb the step and measure the result's sensitivity)
+        self.velocity_hook = None
 
     def set_sgs(self, params, gscale: float, lscale: float, tscale: float):
         """Learned closure: tau = tscale MLP(X), X as les.sgs_features (rounded
@@ -258,6 +261,8 @@ class NSOracle:
                 g = g + fac[3] * q[it[NT - 3]]
             g = re * (g - rhs[c])
             o.solver_3d(self.scu, g, q[n1], *bc[c], self.sigmau)
+            if self.velocity_hook is not None:
+                self.velocity_hook(q[n1])
         # presolve_phi: rhs_p = fac1 q(it1)
         rp = {c: zero_edges(fac[0] * self.q[c][n1]) for c in "uvw"}
         # solve_phi (kmax = 1 without mapping)

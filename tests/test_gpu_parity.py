@@ -313,3 +313,31 @@ def test_solve_c1_helmholtz_and_poisson():
             inner = (slice(1, -1),) * 3
             # the manufactured u is recovered to the conditioning of g
             assert np.abs(got[inner] - u[inner]).max() <= 1e-10
+
+
+def test_all_neumann_poisson_incompatible_rhs():
+    """sigma = 0, Neumann on every face, a right-hand side with a large
+    incompatible part (an offset 100 times its compatible part) on a
+    tanh-2.5 grid whose null eigenvalue LAPACK returns near 1e-11: the
+    device (its own eigen-decomposition) and the oracle (LAPACK's) agree
+    to 1e-11, constant included, because both remove the null component
+    before the transforms."""
+    mf, torch = _mf()
+    shape = (40, 64, 36)
+    gs = (o.grid_uniform(shape[0], 0, 2 * np.pi), o.grid_tanh(shape[1], -1, 1, 2.5),
+          o.grid_uniform(shape[2], 0, np.pi))
+    X, Y, Z = np.meshgrid(*gs, indexing="ij")
+    g = np.asfortranarray(np.sin(X) * np.cos(np.pi * Y) * np.cos(2 * Z) + 100.0)
+    bcs = [np.zeros(s, order="F") for s in ((62, 34, 2), (38, 34, 2), (38, 62, 2))]
+    sc_o = o.SolverCoeffs3D(*gs, (2,) * 6, 6)
+    ref = np.zeros(shape, order="F")
+    o.solver_3d(sc_o, g, ref, *bcs, 0.0)
+    sc = mf.SolverCoeffs3D(*gs, (2,) * 6, 6)
+    u = mf.fortran_zeros(shape)
+    print("run all-neumann poisson %s" % (shape,), flush=True)
+    sc.solve(mf.to_device(g), u, mf.BoundaryCondition(*[mf.to_device(b) for b in bcs]), 0.0, mf.MF_FP64)
+    got = mf.to_host(u)
+    inner = (slice(1, -1),) * 3
+    err = np.abs(got - ref)[inner].max() / np.abs(ref[inner]).max()
+    print("all-neumann poisson: |gpu-oracle64| %.2e" % err, flush=True)
+    assert err <= 1e-11

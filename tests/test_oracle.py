@@ -440,3 +440,37 @@ def test_null_eigenvalue_snapped_to_zero():
             assert c.eigen[i] == 0.0 and abs(raw[i]) > 1e-11 * 0.5
             v = c.vect[:, i]
             assert np.abs(v - v.mean()).max() < 1e-8 * np.abs(v).max()
+
+
+def test_null_component_removal_all_neumann():
+    """sigma = 0 with Neumann faces everywhere: the solve drops c0 = w0.g /
+    w0.1 (w0 the left null vector) from the folded right-hand side.  Since
+    w0^T L = 0, for g = L u_in + 5 the removed part is exactly the offset 5,
+    the solution is u_in up to a constant, and L u equals the compatible part
+    g - c0."""
+    shape = (14, 16, 12)
+    gs = (o.grid_uniform(shape[0], 0, 2), o.grid_tanh(shape[1], -1, 1, 2.0), o.grid_asin(shape[2], 0, 1))
+    sc = o.SolverCoeffs3D(*gs, (2,) * 6, 6)
+    rows = o.null_rows(sc)
+    assert rows is not None
+    rng = np.random.default_rng(4)
+    uin = rng.standard_normal(tuple(s - 2 for s in shape))
+    gi = o.apply_operator(sc, uin, 0.0) + 5.0
+    wx, wy, wz = rows
+    c0 = np.einsum("i,j,k,ijk->", wx, wy, wz, gi) / (wx.sum() * wy.sum() * wz.sum())
+    assert abs(c0 - 5.0) < 1e-9
+    g = np.zeros(shape, order="F")
+    g[1:-1, 1:-1, 1:-1] = gi
+    z = [np.zeros(s, order="F") for s in ((14, 10, 2), (12, 10, 2), (12, 14, 2))]
+    u = np.zeros(shape, order="F")
+    o.solver_3d(sc, g, u, *z, 0.0)
+    ui = u[1:-1, 1:-1, 1:-1]
+    d = ui - uin
+    assert np.abs(d - d.mean()).max() < 1e-9 * np.abs(uin).max()
+    res = o.apply_operator(sc, ui, 0.0) - (gi - c0)
+    assert np.abs(res).max() < 1e-9 * np.abs(gi).max()
+    # a problem that is not singular (sigma = 0 but a Dirichlet axis) is untouched
+    sc2 = o.SolverCoeffs3D(*gs, (2, 2, 1, 1, 2, 2), 6)
+    assert o.null_rows(sc2) is None
+    r = rng.standard_normal(uin.shape)
+    assert o.remove_null_component(sc2, r) is r
