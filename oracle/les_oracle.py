@@ -274,6 +274,17 @@ def loss_and_grad(p, X, T, rounding: bool = True):
     return loss, g
 
 
+def adam_step(p, m, v, g, t: int, lr: float, beta1: float = 0.9, beta2: float = 0.999,
+              eps: float = 1e-8):
+    """One bias-corrected Adam update (Kingma & Ba 2015, Algorithm 1) in FP64:
+    returns the new (p, m, v) after step t >= 1 with gradient g."""
+    m = beta1 * m + (1.0 - beta1) * g
+    v = beta2 * v + (1.0 - beta2) * g * g
+    mh = m / (1.0 - beta1 ** t)
+    vh = v / (1.0 - beta2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
 def init_params(seed: int):
     """Uniform(+-1/sqrt(fan_in)) weights, zero biases (float32)."""
     rng = np.random.default_rng(seed)

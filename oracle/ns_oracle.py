@@ -180,10 +180,22 @@ class NSOracle:
             p, gs, ls, ts = self.sgs
             X = les.sgs_features(du, *self.grids, gs, ls).astype(np.float32)
             Y = les.forward(p, X)[0]
-            for i, c in enumerate("uvw"):
-                for j in range(3):
-                    tau = ts * Y[:, les.pair_col(i, j)].reshape(self.shape, order="F")
-                    out[c] = out[c] - self.der(zero_edges(tau), j)
+            tau = [ts * Y[:, k].reshape(self.shape, order="F") for k in range(6)]
+            f = self.minus_div_tau(tau)
+            for c in "uvw":
+                out[c] = out[c] + f[c]
+        return out
+
+    def minus_div_tau(self, tau):
+        """-sum_j d_j tau_ij for i = u, v, w; tau holds the six columns
+        (xx, yy, zz, xy, xz, yz) as fields, zeroed on the box edges before
+        differentiation (the stress enters as the LES stress does in
+        f(nav,var), class_navier_3D.f90:1083-1134)."""
+        out = {c: np.zeros(self.shape, order="F") for c in "uvw"}
+        for i, c in enumerate("uvw"):
+            for j in range(3):
+                t = np.array(tau[les.pair_col(i, j)], dtype=np.float64, order="F", copy=True)
+                out[c] = out[c] - self.der(zero_edges(t), j)
         return out
 
     # ---- set-up

@@ -9,7 +9,7 @@ the derivative algorithm; the discrete solution refined to ~1e-15 for solves):
 * derivatives and FP64 solves: |gpu - exact| <= max(1e-12, 2 |oracle64 - exact|)
   relative to max|exact|, i.e. 1e-12 or no worse than the FP64 oracle;
 * mixed-precision solves: relative residual of the FP64 operator <= tol; at
-  tol 1e-12 |gpu - exact| <= 1e-10, at tol 1e-14 |gpu - exact| <= max(1e-12,
+  tol 1e-12 |gpu - exact| <= 2e-11, at tol 1e-14 |gpu - exact| <= max(1e-12,
   |oracle64 - exact|): FP64 refinement stalls near cond * eps64, so the mixed
   solve is held to 1e-12 or to the FP64 reference algorithm's own error.
 Every test prints the three distances.
@@ -250,7 +250,7 @@ def test_solve_mixed(shape, bct, sigma, inner):
     grids, sc, g, bcs, uin = _make_problem(shape, bct, sigma, 12)
     ora, exact = _exact_full(sc, g, bcs, sigma, shape)
     exact = _faces_from(sc, exact, bcs)
-    for tol, err_bar in ((1e-12, 1e-10), (1e-14, 1e-12)):
+    for tol, err_bar in ((1e-12, 2e-11), (1e-14, 1e-12)):
         tag = "mixed-%s tol=%g shape=%s bct=%s sigma=%g" % (inner, tol, shape, bct, sigma)
         print("run " + tag, flush=True)
         got, rep = _solve_gpu(mf, grids, bct, g, bcs, sigma, MIXED[inner], tol=tol)

@@ -207,6 +207,64 @@ def test_oracle_mlp_gradient_finite_differences():
         assert abs(fd - g[q]) < 1e-7 + 1e-6 * abs(g[q]), (q, fd, g[q])
 
 
+def _torch_mlp(p, X):
+    """The network written independently in torch (float64): nn.Linear
+    layers with the weights rounded to bfloat16 by torch's own conversion."""
+    import torch
+    sizes = (10, 64, 64, 6)
+    layers, o = [], 0
+    for a, b in zip(sizes[:-1], sizes[1:]):
+        lin = torch.nn.Linear(a, b, dtype=torch.float64)
+        w = torch.tensor(p[o:o + a * b].reshape(b, a), dtype=torch.float32)
+        o += a * b
+        bias = torch.tensor(p[o:o + b], dtype=torch.float32)
+        o += b
+        with torch.no_grad():
+            lin.weight.copy_(w.to(torch.bfloat16).to(torch.float64))
+            lin.bias.copy_(bias.to(torch.float64))
+        layers.append(lin)
+    net = torch.nn.Sequential(layers[0], torch.nn.Tanh(), layers[1], torch.nn.Tanh(), layers[2])
+    return net, net(torch.tensor(X, dtype=torch.float64))
+
+
+def test_oracle_mlp_forward_and_gradient_vs_torch():
+    """les_oracle.forward / loss_and_grad against torch autograd: a
+    transposed weight, a wrong parameter order or a wrong rounding in the
+    oracle fails here (finite differences of the oracle against itself
+    cannot see those)."""
+    import torch
+    p = L.init_params(12)
+    p[L.NPARAM - 6:] = np.float32(0.07)
+    X, T = _mlp_data(257, 5)
+    net, Yt = _torch_mlp(p, X)
+    Y = L.forward(p, X)[0]
+    assert np.abs(Y - Yt.detach().numpy()).max() < 1e-13 * np.abs(Y).max()
+    loss_t = ((Yt - torch.tensor(T, dtype=torch.float64)) ** 2).sum() / len(X)
+    loss_t.backward()
+    gt = np.concatenate([t.grad.numpy().ravel() for lin in (net[0], net[2], net[4])
+                         for t in (lin.weight, lin.bias)])
+    loss, g = L.loss_and_grad(p, X, T)
+    assert abs(loss - loss_t.item()) < 1e-13 * loss
+    assert np.abs(g - gt).max() < 1e-12 * np.abs(gt).max()
+
+
+def test_oracle_adam_vs_torch():
+    """les_oracle.adam_step against torch.optim.Adam (same betas, eps, lr)
+    over three steps with varying gradients."""
+    import torch
+    rng = np.random.default_rng(6)
+    p0 = rng.standard_normal(50)
+    pt = torch.tensor(p0.copy(), requires_grad=True)
+    opt = torch.optim.Adam([pt], lr=3e-3, betas=(0.9, 0.999), eps=1e-8)
+    p, m, v = p0.copy(), np.zeros(50), np.zeros(50)
+    for t in range(1, 4):
+        g = rng.standard_normal(50) * 10.0 ** (-t)
+        pt.grad = torch.tensor(g)
+        opt.step()
+        p, m, v = L.adam_step(p, m, v, g, t, 3e-3)
+        assert np.abs(p - pt.detach().numpy()).max() < 1e-15
+
+
 def test_oracle_mlp_rounding_is_weights_only():
     p = L.init_params(8)
     X, T = _mlp_data(33, 2)
@@ -345,9 +403,7 @@ def test_gpu_mlp_adam_matches_oracle_and_trains():
         loss = net.adam_step(Xd, Td, lr)
         lref, g = L.loss_and_grad(pr.astype(np.float32), X, T)
         assert abs(loss - lref) < 1e-4 * lref
-        m = 0.9 * m + 0.1 * g
-        v = 0.999 * v + 0.001 * g * g
-        pr = pr - lr * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+        pr, m, v = L.adam_step(pr, m, v, g, t, lr)
     # a parameter with a tiny gradient may move by up to lr per step in
     # either direction under Adam; compare where the gradient is not tiny
     pd = net.params()

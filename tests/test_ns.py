@@ -252,10 +252,10 @@ def test_ns_steps_match_oracle(tou, top):
         scale = max(np.abs(b).max(), 1e-30)
         err = np.abs(a - b).max() / scale
         print("  %s: max|gpu-oracle|/max|oracle| = %.2e" % (name, err), flush=True)
-        assert err < 1e-9, (name, err)
+        assert err < 1e-10, (name, err)
     t, g = dev.info()
     assert abs(t - steps * dt) < 1e-12
-    assert abs(g - orc.flow.g) < 1e-9 * abs(orc.flow.g)
+    assert abs(g - orc.flow.g) < 1e-10 * abs(orc.flow.g)
 
 
 @pytest.mark.gpu
@@ -276,7 +276,7 @@ def test_ns_mixed_velocity_solves(prec):
         xa, ya = mf.to_host(x), mf.to_host(y)
         err = np.abs(xa - ya).max() / max(np.abs(xa).max(), 1e-30)
         print("  %s: mixed vs fp64 %.2e" % (name, err), flush=True)
-        assert err < 1e-9, (name, err)
+        assert err < 1e-10, (name, err)
 
 
 @pytest.mark.gpu
@@ -306,7 +306,7 @@ def test_ns_les_matches_oracle(les_type, cs):
     # relative error, large where the flow is nearly two-dimensional (the
     # channel's base shear): nu itself (proportional to s3) agrees to about
     # sqrt(eps) of its largest value, the flow it drives to 1e-7
-    tol = {"u": 1e-9, "v": 1e-9, "w": 1e-9, "p": 1e-9, "nu": 1e-9}
+    tol = {"u": 1e-10, "v": 1e-10, "w": 1e-10, "p": 1e-10, "nu": 1e-10}
     if les_type == 3:
         tol = {"u": 1e-7, "v": 1e-7, "w": 1e-7, "p": 1e-7, "nu": 1e-6}
     errs = {}
@@ -314,3 +314,25 @@ def test_ns_les_matches_oracle(les_type, cs):
         errs[name] = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
         print("  %s: max|gpu-oracle|/max|oracle| = %.2e" % (name, errs[name]), flush=True)
     assert all(errs[k] < tol[k] for k in errs), errs
+
+
+def test_oracle_minus_div_tau_closed_form():
+    """The learned closure's force -sum_j d_j tau_ij on cubic stress fields,
+    which every compact row differentiates exactly: equal to the analytic
+    divergence at interior points (any swapped column, index or sign
+    fails)."""
+    shape = (12, 14, 10)
+    gx, gy, gz = o.grid_uniform(12, 0, 1), o.grid_tanh(14, -1, 1, 1.5), o.grid_asin(10, 0, 2)
+    s = ns.NSOracle(gx, gy, gz, 100.0, 1e-3)
+    X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+    # columns xx, yy, zz, xy, xz, yz and their gradients (d/dx, d/dy, d/dz)
+    tau = [X ** 3 + Y, Y ** 2 * Z, Z ** 3 - X * Y, X * Y * Z, X ** 2 * Z, Y ** 3 + X]
+    grad = [(3 * X ** 2, 1 + 0 * X, 0 * X), (0 * X, 2 * Y * Z, Y ** 2), (-Y, -X, 3 * Z ** 2),
+            (Y * Z, X * Z, X * Y), (2 * X * Z, 0 * X, X ** 2), (1 + 0 * X, 3 * Y ** 2, 0 * X)]
+    cols = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (0, 2): 4, (1, 2): 5}
+    f = s.minus_div_tau([np.asfortranarray(t) for t in tau])
+    inner = (slice(1, -1),) * 3
+    for i, c in enumerate("uvw"):
+        ref = -sum(grad[cols[(min(i, j), max(i, j))]][j] for j in range(3))
+        err = np.abs(f[c] - ref)[inner].max() / np.abs(ref[inner]).max()
+        assert err < 1e-11, (c, err)
