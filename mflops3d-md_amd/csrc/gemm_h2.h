@@ -23,6 +23,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "gemm.h"
 
 namespace mf {
@@ -44,6 +46,131 @@ __device__ __forceinline__ void split2h(float xs, _Float16 &p0, _Float16 &p1) {
   p1 = (_Float16)(xs - (float)p0);
 }
 
+// Epilogue of the fp16x2 transforms (as gemm_bf3_kernel's): acc[mi][ni][r]
+// holds D(m, n), m = m0 + wm + 16 mi + lane % 16, n = n0 + wn + 4 (lane / 16)
+// + 16 ni + r, scaled by 1 / unscale (an exact power of two).  EPI_STORE and
+// EPI_DIVIDE store FP32 (divide: by lx + ly + lz + sigma, 0 at the null
+// mode) and fold the block's largest stored magnitude into fmax_out;
+// EPI_SCATTER / EPI_SCATTER_ADD write or add into the interior of u (FP64).
+// amax receives the largest |acc| stored (before unscale).
+//
+// Addressing: one 64-bit base per lane, then offsets that are wave-uniform
+// per (mi, ni, r) (scalar arithmetic); bound checks only in waves whose 64 x
+// 64 tile crosses M or N (a wave-uniform test picks the checked body).
+template <typename TC, int BM, int BN, int EPI>
+__device__ __forceinline__ void h2_store(f4_t (&acc)[4][4], float unscale, long m0, long n0,
+                                         int wm, int wn, int lane, TC *__restrict__ C, long ldc,
+                                         int M, int N, const EpiArgs &ep, float &amax) {
+  const int lm = lane & 15, ln = 4 * (lane >> 4);
+  const long mw = m0 + wm, nw = n0 + wn;  // wave-uniform
+  const bool full = mw + 64 <= M && nw + 64 <= N;
+  if (EPI == EPI_STORE || EPI == EPI_DIVIDE) {
+    TC *pl = C + (mw + lm) + ldc * (nw + ln);
+    float lzf[4][4];
+    bool nullz[4][4];
+    if (EPI == EPI_DIVIDE) {
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long n = nw + ln + ni * 16 + r;
+          const double lz = ep.lz[n < N ? n : N - 1];
+          lzf[ni][r] = (float)lz;
+          nullz[ni][r] = fabs(lz) < 1e-11;
+        }
+    }
+    auto body = [&](auto checked) {
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) {
+        const long m = mw + 16 * mi + lm;
+        if (decltype(checked)::value && m >= M) continue;
+        float lxy = 0.0f;
+        bool null_xy = false;
+        if (EPI == EPI_DIVIDE) {
+          const int i1 = (int)(m % ep.n1), i2 = (int)(m / ep.n1);
+          lxy = (float)(ep.lx[i1] + ep.ly[i2] + ep.sigma);
+          null_xy = fmax(fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2])), fabs(ep.sigma)) < 1e-11;
+        }
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (decltype(checked)::value && nw + ln + 16 * ni + r >= N) continue;
+            float v = acc[mi][ni][r];
+            if (EPI == EPI_DIVIDE)
+              v = (null_xy && nullz[ni][r]) ? 0.0f : v * __builtin_amdgcn_rcpf(lxy + lzf[ni][r]);
+            amax = fmaxf(amax, fabsf(v));
+            pl[16 * mi + ldc * (16 * ni + r)] = (TC)(v * unscale);
+          }
+      }
+    };
+    if (full) body(std::false_type{});
+    else body(std::true_type{});
+  } else {
+    // u[(i+1) + nx((j+1) + ny(k+1))] for m = i, n = j + n2 k.  With nb = n0 +
+    // wn + 4 (lane / 16) = jb + n2 kb and dn = 16 ni + r < 64 <= n2, n = nb +
+    // dn wraps at most once: the offset is nx dn past the lane's base, plus 2
+    // nx (the two boundary rows of the next plane) when jb + dn >= n2.
+    const long nb = nw + ln;
+    const int jb = (int)(nb % ep.n2), kb = (int)(nb / ep.n2);
+    const long nx = ep.nx, nxy = (long)ep.nx * ep.ny;
+    double *ul = ep.u + (mw + lm + 1) + nx * (jb + 1) + nxy * (kb + 1);
+    auto body = [&](auto checked) {
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int dn = 16 * ni + r;
+          if (decltype(checked)::value && nb + dn >= N) continue;
+          long off;
+          if (ep.n2 >= 64) {
+            off = nx * dn + (jb + dn >= ep.n2 ? 2 * nx : 0);
+          } else {
+            const int q = (jb + dn) / ep.n2;  // planes crossed
+            off = nx * dn + 2 * nx * q;
+          }
+#pragma unroll
+          for (int mi = 0; mi < 4; ++mi) {
+            if (decltype(checked)::value && mw + 16 * mi + lm >= M) continue;
+            double *pu = ul + 16 * mi + off;
+            const double v = (double)(acc[mi][ni][r] * unscale);
+            if (EPI == EPI_SCATTER_ADD) *pu = *pu + v;
+            else *pu = v;
+          }
+        }
+    };
+    if (full) body(std::false_type{});
+    else body(std::true_type{});
+  }
+}
+
+// h2_store, then the block's largest stored magnitude into fmax_out (one
+// atomic per block).  Every thread of the block must call it.
+template <typename TC, int BM, int BN, int EPI>
+__device__ __forceinline__ void h2_epilogue(f4_t (&acc)[4][4], float unscale, long m0, long n0,
+                                            int wm, int wn, int lane, int wave, int tid,
+                                            TC *__restrict__ C, long ldc, int M, int N,
+                                            const EpiArgs &ep, unsigned *__restrict__ fmax_out) {
+  constexpr int NT = 256;
+  float amax = 0.0f;  // largest |acc| stored (before unscale)
+  h2_store<TC, BM, BN, EPI>(acc, unscale, m0, n0, wm, wn, lane, C, ldc, M, N, ep, amax);
+  if ((EPI == EPI_STORE || EPI == EPI_DIVIDE) && fmax_out) {
+    // block maximum, then one atomic per block
+    __shared__ float bmax[NT / 64];
+    float vmax = amax * unscale;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+    if (lane == 0) bmax[wave] = vmax;
+    __syncthreads();
+    if (tid == 0) {
+      float b = bmax[0];
+#pragma unroll
+      for (int q = 1; q < NT / 64; ++q) b = fmaxf(b, bmax[q]);
+      atomicMax(fmax_out, __float_as_uint(b));
+    }
+  }
+}
+
 // Wsplit: fp16 [2][KP/8][WROWS][8] of the scaled matrix (wscale applied).
 // fmax_in: the field's largest magnitude (float bits); fmax_out: NULL or the
 // scalar receiving the largest stored magnitude (EPI_STORE / EPI_DIVIDE).
@@ -62,7 +189,7 @@ __global__ void __launch_bounds__(256, 2)
   constexpr int NG = RF * (BK / 8) / NT;  // groups of 8 field values per thread
   static_assert(BM * BN == NT * 64 && NG * NT == RF * (BK / 8), "tile");
   __shared__ __attribute__((aligned(16))) _Float16 Fs[2][2][RF][BK + PAD];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const long m0 = (long)blockIdx.x * BM, n0 = (long)blockIdx.y * BN;
   F += (long)blockIdx.z * sF;
   if (EPI == EPI_STORE || EPI == EPI_DIVIDE) C += (long)blockIdx.z * sC;
@@ -177,109 +304,8 @@ __global__ void __launch_bounds__(256, 2)
     loadW(min(t + 2, nk2 - 1), wA);
     step(1, wB);
   }
-  // ---- epilogue (as gemm_bf3_kernel), values unscaled by 1 / (fs wscale),
-  // an exact power of two
-  const float unscale = 1.0f / (fs * wscale);
-  float vmax = 0.0f;
-  int i1 = 0, i2 = 0;
-  if (EPI == EPI_DIVIDE) {
-    const int mb = (int)(m0 + wm + (lane & 15));
-    i1 = mb % ep.n1;
-    i2 = mb / ep.n1;
-  }
-  int jb = 0, kb = 0;
-  if (EPI == EPI_SCATTER || EPI == EPI_SCATTER_ADD) {
-    const int nb = (int)(n0 + wn + 4 * (lane >> 4));
-    jb = nb % ep.n2;
-    kb = nb / ep.n2;
-  }
-  float lzf[4][4];
-  bool nullz[4][4];
-  if (EPI == EPI_DIVIDE) {
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long n = n0 + wn + 4 * (lane >> 4) + ni * 16 + r;
-        const double lz = ep.lz[n < N ? n : N - 1];
-        lzf[ni][r] = (float)lz;
-        nullz[ni][r] = fabs(lz) < 1e-11;
-      }
-  }
-  auto uidx = [&](long m, int dn) {
-    int j = jb + dn, k = kb;
-    while (j >= ep.n2) { j -= ep.n2; ++k; }
-    return (m + 1) + (long)ep.nx * ((j + 1) + (long)ep.ny * (k + 1));
-  };
-#pragma unroll
-  for (int mi = 0; mi < 4; ++mi) {
-    const long m = m0 + wm + mi * 16 + (lane & 15);
-    if (mi > 0 && EPI == EPI_DIVIDE) {
-      i1 += 16;
-      while (i1 >= ep.n1) { i1 -= ep.n1; ++i2; }
-    }
-    if (m >= M) continue;
-    float lxy = 0.0f;
-    bool null_xy = false;
-    if (EPI == EPI_DIVIDE) {
-      lxy = (float)(ep.lx[i1] + ep.ly[i2] + ep.sigma);
-      null_xy = fmax(fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2])), fabs(ep.sigma)) < 1e-11;
-    }
-    if (EPI == EPI_SCATTER_ADD) {
-      double old[4][4];
-#pragma unroll
-      for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int dn = ni * 16 + r;
-          const bool ok = n0 + wn + 4 * (lane >> 4) + dn < N;
-          old[ni][r] = ok ? ep.u[uidx(m, dn)] : 0.0;
-        }
-#pragma unroll
-      for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int dn = ni * 16 + r;
-          if (n0 + wn + 4 * (lane >> 4) + dn >= N) continue;
-          ep.u[uidx(m, dn)] = old[ni][r] + (double)(acc[mi][ni][r] * unscale);
-        }
-      continue;
-    }
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int dn = ni * 16 + r;
-        const long n = n0 + wn + 4 * (lane >> 4) + dn;
-        if (n >= N) continue;
-        const float v = acc[mi][ni][r] * unscale;
-        if (EPI == EPI_STORE) {
-          C[m + ldc * n] = (TC)v;
-          vmax = fmaxf(vmax, fabsf(v));
-        } else if (EPI == EPI_DIVIDE) {
-          const float inv = __builtin_amdgcn_rcpf(lxy + lzf[ni][r]);
-          const float o = (null_xy && nullz[ni][r]) ? 0.0f : v * inv;
-          C[m + ldc * n] = (TC)o;
-          vmax = fmaxf(vmax, fabsf(o));
-        } else {
-          ep.u[uidx(m, dn)] = (double)v;
-        }
-      }
-  }
-  if ((EPI == EPI_STORE || EPI == EPI_DIVIDE) && fmax_out) {
-    // block maximum, then one atomic per block
-    __shared__ float bmax[NT / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
-    if (lane == 0) bmax[wave] = vmax;
-    __syncthreads();
-    if (tid == 0) {
-      float b = bmax[0];
-#pragma unroll
-      for (int q = 1; q < NT / 64; ++q) b = fmaxf(b, bmax[q]);
-      atomicMax(fmax_out, __float_as_uint(b));
-    }
-  }
+  h2_epilogue<TC, BM, BN, EPI>(acc, 1.0f / (fs * wscale), m0, n0, wm, wn, lane, wave, tid, C, ldc,
+                               M, N, ep, fmax_out);
 }
 
 }  // namespace mf

@@ -28,6 +28,7 @@
 #include "gemm.h"
 #include "gemm_bf3.h"
 #include "gemm_h2.h"
+#include "gemm_h2r.h"
 #include "gemm_h2z.h"
 #include "influence.h"
 #include "internal.h"
@@ -228,6 +229,8 @@ struct mf_solver3d {
   // singular problems (sigma = 0, a zero eigenvalue on every axis): the
   // null rows w0x | w0y | w0z of inv(V), 1 / (sum w0x sum w0y sum w0z), and
   // n3 plane partials
+  bool h2_resident = true;  // fp16x2 transforms with the eigenvectors in registers (MF_H2_RESIDENT=0: off)
+  int ncu = 256;        // compute units of the device the handle was created on
   bool null_mode = false;
   double *w0 = nullptr, *npart = nullptr;
   double inv_wsum = 0.0;
@@ -519,6 +522,13 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
           else if (v != 0.0) zs_ok = false;
         }
       }
+  }
+  if (const char *env = std::getenv("MF_H2_RESIDENT")) h->h2_resident = std::atoi(env) != 0;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      h->ncu = cus;
   }
   size_t nparts_zs = 0;
   if (zs_ok) {
@@ -867,6 +877,21 @@ void launch_h2(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W
                int KP, int WROWS, TC *C, long ldc, long sC, int M, int N, int K, int batch,
                const EpiArgs &ep, int slot_in, int slot_out, bool zero0, hipStream_t st) {
   const int t = gemm_tile(WLEFT ? M : N);
+  if (WLEFT && t == 256 && WROWS == 256 && KP <= mf::H2R_KP && h->h2_resident) {
+    // eigenvector matrix resident in registers, persistent blocks (gemm_h2r.h);
+    // x transforms only: 43.6 us against 45.6 us for gemm_h2_kernel at 254^3,
+    // while the y orientation is slower there (43.5 against 37.0 us)
+    const long fr = WLEFT ? N : M;
+    const long tiles = (fr + mf::H2R_RF - 1) / mf::H2R_RF * batch;
+    const int grid = (int)std::min<long>(tiles, h->ncu);
+    ProfScope ps(h, P_T32, 2.0 * M * N * K * batch, st);
+    hipLaunchKernelGGL((mf::gemm_h2r_kernel<TC, WLEFT, EPI>), dim3(grid), dim3(512), 0, st,
+                       (const float *)F, ldf, sF, W, KP, C, ldc, sC, M, N, K, batch, ep,
+                       (const unsigned *)(h->fmx + slot_in), wsc,
+                       slot_out >= 0 ? h->fmx + slot_out : (unsigned *)nullptr,
+                       zero0 ? h->fmx : (unsigned *)nullptr);
+    return;
+  }
 #define MF_H2(BM, BN)                                                                      \
   launch_h2_tile<TF, TC, BM, BN, WLEFT, EPI>(h, F, ldf, sF, W, wsc, KP, WROWS, C, ldc, sC, M, N, \
                                              K, batch, ep, slot_in, slot_out, zero0, st)
