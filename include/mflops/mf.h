@@ -292,6 +292,22 @@ int mf_ns_info(const mf_ns *ns, double *time, double *flow_force, void *stream);
  *    as the reference does) from the latest level's velocity gradients and
  *    adds sum_j d/dx_j (nu (du_i/dx_j + du_j/dx_i)) to the force of u_i. */
 int mf_ns_set_les(mf_ns *ns, int les_type, double c, void *stream);
+/* <- pt and nsubite (command_line.f90:22,32; navier_initialization
+ *    class_navier_3D.f90:2133,2136): projection type 1 (default:
+ *    non-incremental, tangential boundary gradient), 2 (incremental: the
+ *    velocity sees grad extrapol_p(p)), 3 (velocity correction: pressure
+ *    first, rot-rot Neumann term, add_boundary_rotrot :793-846) or 4
+ *    (incremental velocity correction), and at most nsubite sub-iterations
+ *    per step (navier3d.f90:56-147; from the second on every extrapolation
+ *    returns the current iterate and testconv :366-412 stops the loop when
+ *    the relative changes of |u| and of p fall below 1e-9; each test
+ *    synchronises the stream).  Clears the stored presolve right-hand sides
+ *    (rhs_u(1:2), zero after navier_initialization); call before stepping.
+ *    MF_ERR_ARG for pt outside 1..4 or nsubite < 1. */
+int mf_ns_set_scheme(mf_ns *ns, int pt, int nsubite, void *stream);
+/* Sub-iterations made by the last step and the velocity and pressure
+ * changes of its last convergence test (0 when it made one). */
+int mf_ns_subiterations(const mf_ns *ns, int *count, double *dvel, double *dp);
 /* nu of the last step with LES into a device field (<- the les_nu output
  * field, class_navier_3D.f90:101). */
 int mf_ns_les_nu(const mf_ns *ns, double *nu, void *stream);

@@ -1,9 +1,12 @@
 // Prediction-projection Navier-Stokes time step on the device
-// (navier3d.f90:48-286, class_navier_3D.f90), single domain, with the
-// reference's defaults: skew-symmetric nonlinear term (nlt=2; nlt=1 the
-// advective form), non-incremental rotational pressure correction with the
-// tangential boundary-gradient correction (pt=1), BDF order tou and pressure
-// extrapolation order top-1 (1..3 each), no mapping, one sub-iteration.
+// (navier3d.f90:48-286, class_navier_3D.f90), single domain, no mapping:
+// skew-symmetric (nlt=2, the default) or advective (nlt=1) nonlinear term,
+// BDF order tou and pressure extrapolation order top-1 (1..3 each),
+// projection type pt = 1 (the default: non-incremental rotational correction
+// with the tangential boundary-gradient correction), 2 (incremental), 3
+// (velocity correction with the rot-rot pressure boundary term) or 4
+// (incremental velocity correction), and up to nsubite sub-iterations per
+// step, stopped by the reference's convergence test.
 //
 // All time levels stay in HBM.  The nonlinear term of a time level is
 // computed once, when the level is first needed, and cached (the reference
@@ -147,35 +150,212 @@ __global__ void __launch_bounds__(1024) flow_kernel(const double *__restrict__ p
 
 struct RhsArgs {
   const double *q0, *q1, *q2;     // q^n, q^n-1, q^n-2
-  const double *n0, *n1, *n2;     // N^n, N^n-1, N^n-2
+  const double *n0, *n1, *n2;     // extrapolated levels of N (unused ones have c = 0)
   double f2, f3, f4;              // fac(2..4)
   double c0, c1, c2;              // extrapolation coefficients
   const double *force;            // or null
   const double *gflow;            // uniform force (device scalar) or null
   const double *les;              // -les_f: sum_j d_j (nu (du_ij + du_ji)), or null
+  const double *rhs_in;           // a stored rhs1 to use instead of computing it, or null
+  double *rhs_out;                // where to keep the computed rhs1, or null
+  const double *dp;               // pressure-gradient component, or null
   double re;
-  double *out;
+  double *out;                    // velocity right-hand side, or null
   long N;
   Dims d;
 };
 
-// g = Re (fac2 q^n + fac3 q^n-1 + fac4 q^n-2 - rhs), rhs = zero_edges(f - zero_edges(extrapol N))
+// rhs1 = zero_edges(f - zero_edges(extrapol N))   (navier_presolve_u/v/w)
+// g = Re (fac2 q^n + fac3 q^n-1 + fac4 q^n-2 + dp - rhs1)   (navier_solve_u/v/w)
 __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.N) return;
-  const bool edge = on_edge(t, a.d);
-  double ne = a.c0 * a.n0[t];
-  if (a.c1 != 0.0) ne += a.c1 * a.n1[t];
-  if (a.c2 != 0.0) ne += a.c2 * a.n2[t];
-  if (edge) ne = 0.0;
-  double f = a.force ? a.force[t] : 0.0;
-  if (a.gflow) f += a.gflow[0];
-  if (a.les) f += a.les[t];
-  double rhs = f - ne;
-  if (edge) rhs = 0.0;
+  double rhs;
+  if (a.rhs_in) {
+    rhs = a.rhs_in[t];
+  } else {
+    const bool edge = on_edge(t, a.d);
+    double ne = a.c0 * a.n0[t];
+    if (a.c1 != 0.0) ne += a.c1 * a.n1[t];
+    if (a.c2 != 0.0) ne += a.c2 * a.n2[t];
+    if (edge) ne = 0.0;
+    double f = a.force ? a.force[t] : 0.0;
+    if (a.gflow) f += a.gflow[0];
+    if (a.les) f += a.les[t];
+    rhs = f - ne;
+    if (edge) rhs = 0.0;
+    if (a.rhs_out) a.rhs_out[t] = rhs;
+  }
+  if (!a.out) return;
   double g = a.f2 * a.q0[t] + a.f3 * a.q1[t];
   if (a.f4 != 0.0) g += a.f4 * a.q2[t];
+  if (a.dp) g += a.dp[t];
   a.out[t] = a.re * (g - rhs);
+}
+
+// out = zero_edges(c0 a0 + c1 a1 + c2 a2) (navier_extrapol; a level with
+// c = 0 is not read)
+__global__ void __launch_bounds__(256) extrap_kernel(const double *__restrict__ a0,
+                                                     const double *__restrict__ a1,
+                                                     const double *__restrict__ a2, double c0,
+                                                     double c1, double c2,
+                                                     double *__restrict__ out, long N, Dims d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  double e = 0.0;
+  if (c0 != 0.0) e = c0 * a0[t];
+  if (c1 != 0.0) e += c1 * a1[t];
+  if (c2 != 0.0) e += c2 * a2[t];
+  out[t] = on_edge(t, d) ? 0.0 : e;
+}
+
+// Right-hand sides rhs_p of the pressure equation (navier_phi_rhs,
+// class_navier_3D.f90:1961-2007), zero on the edges, for the three components:
+//   mode 0 (pt <= 2):              fac1 q(it1)
+//   mode 3 (pt = 3):               rhs1 - fac2 q^n - fac3 q^n-1 - fac4 q^n-2
+//   mode 4 (pt = 4, subite = 1):   mode 3 - rhs2 + fac1 q^n + fac2 q^n-1 + fac3 q^n-2 + fac4 q(it1)
+//   mode 5 (pt = 4, subite > 1):   rhs1 - rhs2 + fac1 q(it1)
+struct RpArgs {
+  const double *qi[3], *q0[3], *q1[3], *q2[3];  // levels it(1), it(nt), it(nt-1), it(nt-2)
+  const double *r1[3], *r2[3];
+  double f1, f2, f3, f4;
+  int mode;
+  double *out[3];
+  long N;
+  Dims d;
+};
+
+__global__ void __launch_bounds__(256) rp_kernel(RpArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.N) return;
+  const bool e = on_edge(t, a.d);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double r;
+    if (a.mode == 0) {
+      r = a.f1 * a.qi[c][t];
+    } else if (a.mode == 5) {
+      r = a.r1[c][t] - a.r2[c][t];
+      r = r + a.f1 * a.qi[c][t];
+    } else {
+      const double x0 = a.q0[c][t], x1 = a.q1[c][t], x2 = a.q2[c][t];
+      r = a.r1[c][t] - a.f2 * x0;
+      r = r - a.f3 * x1;
+      r = r - a.f4 * x2;
+      if (a.mode == 4) {
+        r = r - a.r2[c][t];
+        r = r + a.f1 * x0;
+        r = r + a.f2 * x1;
+        r = r + a.f3 * x2;
+        r = r + a.f4 * a.qi[c][t];
+      }
+    }
+    a.out[c][t] = e ? 0.0 : r;
+  }
+}
+
+// add_boundary_rotrot (class_navier_3D.f90:793-846) on the two faces normal
+// to axis `axis`: bcphi -= (d_a(d_b e_b + d_c e_c) - dd_b e_a - dd_c e_a)/Re,
+// with t2 = d_a(...), db = dd_b e_a, dc = dd_c e_a given as fields
+__global__ void __launch_bounds__(256) rotrot_kernel(int axis, const double *__restrict__ t2,
+                                                     const double *__restrict__ db,
+                                                     const double *__restrict__ dc, double re,
+                                                     double *__restrict__ bcphi, Dims d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const long fx = (long)(ny - 2) * (nz - 2), fy = (long)(nx - 2) * (nz - 2), fz = (long)(nx - 2) * (ny - 2);
+  const long n = axis == 0 ? fx : (axis == 1 ? fy : fz);
+  if (t >= 2 * n) return;
+  const int side = (int)(t / n);
+  const long r = t % n;
+  long i, j, k, off;
+  if (axis == 0) {
+    j = 1 + r % (ny - 2); k = 1 + r / (ny - 2); i = side ? nx - 1 : 0; off = t;
+  } else if (axis == 1) {
+    i = 1 + r % (nx - 2); k = 1 + r / (nx - 2); j = side ? ny - 1 : 0; off = 2 * fx + t;
+  } else {
+    i = 1 + r % (nx - 2); j = 1 + r / (nx - 2); k = side ? nz - 1 : 0; off = 2 * (fx + fy) + t;
+  }
+  const long q = i + (long)nx * (j + (long)ny * k);
+  bcphi[off] -= (t2[q] - db[q] - dc[q]) / re;
+}
+
+// Partial sums of the sub-iteration test (testconv, navier3d.f90:366-412) per
+// block: W |q|^2, W |q - q_sub|^2 and W (p - p_sub)
+__global__ void __launch_bounds__(256) conv1_kernel(const double *__restrict__ W,
+                                                    const double *__restrict__ u,
+                                                    const double *__restrict__ v,
+                                                    const double *__restrict__ w,
+                                                    const double *__restrict__ su,
+                                                    const double *__restrict__ sv,
+                                                    const double *__restrict__ sw,
+                                                    const double *__restrict__ p,
+                                                    const double *__restrict__ sp, long N,
+                                                    double *__restrict__ part) {
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long)gridDim.x * 256) {
+    const double wt = W[t];
+    const double x = u[t], y = v[t], z = w[t];
+    const double dx = su[t] - x, dy = sv[t] - y, dz = sw[t] - z;
+    a += wt * (x * x + y * y + z * z);
+    b += wt * (dx * dx + dy * dy + dz * dz);
+    c += wt * (p[t] - sp[t]);
+  }
+  __shared__ double sh[3][256];
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  sh[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int m = 0; m < 3; ++m) sh[m][threadIdx.x] += sh[m][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// second pass: with m = sum W (p - p_sub) / sum W, partial sums of
+// W (p_sub + m)^2 and W (p - p_sub - m)^2
+__global__ void __launch_bounds__(256) conv2_kernel(const double *__restrict__ W,
+                                                    const double *__restrict__ p,
+                                                    const double *__restrict__ sp,
+                                                    const double *__restrict__ sums, double vol,
+                                                    long N, double *__restrict__ part) {
+  const double m = sums[2] / vol;
+  double a = 0.0, b = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long)gridDim.x * 256) {
+    const double wt = W[t];
+    const double s = sp[t] + m, e = p[t] - s;
+    a += wt * s * s;
+    b += wt * e * e;
+  }
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int k = 0; k < 2; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) part[threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// out[q] = sum of part[q np .. q np + np - 1] in a fixed tree order, q < nq
+// (one block of 1024 threads per quantity)
+__global__ void __launch_bounds__(1024) psum_kernel(const double *__restrict__ part, int np,
+                                                    double *__restrict__ out) {
+  __shared__ double sh[1024];
+  const double *pp = part + (long)blockIdx.x * np;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < np; b += 1024) v += pp[b];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
 struct BcArgs {
@@ -225,20 +405,6 @@ __global__ void __launch_bounds__(256) bc_kernel(BcArgs a) {
   a.bcw[comp][off] = v;
 }
 
-// rp_c = zero_edges(fac1 q_c)
-__global__ void __launch_bounds__(256) rp_kernel(const double *__restrict__ u,
-                                                 const double *__restrict__ v,
-                                                 const double *__restrict__ w, double fac1,
-                                                 double *__restrict__ ru, double *__restrict__ rv,
-                                                 double *__restrict__ rw, long N, Dims d) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N) return;
-  const bool e = on_edge(t, d);
-  ru[t] = e ? 0.0 : fac1 * u[t];
-  rv[t] = e ? 0.0 : fac1 * v[t];
-  rw[t] = e ? 0.0 : fac1 * w[t];
-}
-
 __global__ void __launch_bounds__(256) sum3_kernel(const double *__restrict__ a,
                                                    const double *__restrict__ b,
                                                    const double *__restrict__ c,
@@ -246,6 +412,14 @@ __global__ void __launch_bounds__(256) sum3_kernel(const double *__restrict__ a,
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
   out[t] = a[t] + b[t] + c[t];
+}
+
+__global__ void __launch_bounds__(256) sum2_kernel(const double *__restrict__ a,
+                                                   const double *__restrict__ b,
+                                                   double *__restrict__ out, long N) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  out[t] = a[t] + b[t];
 }
 
 // bcphi faces: rhs_p at the face minus fac1 times the normal velocity bc
@@ -333,24 +507,31 @@ __global__ void __launch_bounds__(256) les_acc_kernel(const double *__restrict__
   }
 }
 
-__global__ void __launch_bounds__(256) proj_kernel(const double *__restrict__ phi,
-                                                   const double *__restrict__ fphi,
-                                                   const double *__restrict__ ru,
-                                                   const double *__restrict__ rv,
-                                                   const double *__restrict__ rw,
-                                                   const double *__restrict__ dx,
-                                                   const double *__restrict__ dy,
-                                                   const double *__restrict__ dz, double fac1,
-                                                   double re, double *__restrict__ p,
-                                                   double *__restrict__ u, double *__restrict__ v,
-                                                   double *__restrict__ w, long N, Dims d) {
+struct ProjArgs {
+  const double *phi;
+  const double *pe;        // extrapol_p(p) (pt = 2, 4) or null
+  const double *dv;        // divergence term or null
+  int dv_fac1;             // 1: dv is div(fac1 u*) (pt <= 2), 0: div(extrapol_p(q)) (pt = 4)
+  const double *r[3];      // rhs_p
+  const double *g[3];      // grad phi
+  double fac1, re;
+  double *p, *q[3];
+  long N;
+  Dims d;
+};
+
+// navier_projection (:1847-1959): p = zero_edges(phi [+ pe] [- div/Re]);
+// q_c = zero_edges((rhs_pc - d_c phi)/fac1)
+__global__ void __launch_bounds__(256) proj_kernel(ProjArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N) return;
-  const bool e = on_edge(t, d);
-  p[t] = e ? 0.0 : phi[t] - (fphi[t] / fac1) / re;
-  u[t] = e ? 0.0 : (ru[t] - dx[t]) / fac1;
-  v[t] = e ? 0.0 : (rv[t] - dy[t]) / fac1;
-  w[t] = e ? 0.0 : (rw[t] - dz[t]) / fac1;
+  if (t >= a.N) return;
+  const bool e = on_edge(t, a.d);
+  double pv = a.phi[t];
+  if (a.pe) pv = pv + a.pe[t];
+  if (a.dv) pv = pv - (a.dv_fac1 ? (a.dv[t] / a.fac1) / a.re : a.dv[t] / a.re);
+  a.p[t] = e ? 0.0 : pv;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) a.q[c][t] = e ? 0.0 : (a.r[c][t] - a.g[c][t]) / a.fac1;
 }
 
 __global__ void __launch_bounds__(256) weights_kernel(const double *__restrict__ gx,
@@ -411,6 +592,17 @@ struct mf_ns {
   double sgs_g = 1.0, sgs_l = 1.0, sgs_t = 1.0;
   void *sgsmem = nullptr;
   float *sgsX = nullptr, *sgsY = nullptr;
+  // projection type and sub-iterations (navier_initialization :2133,2136):
+  // rhs1/rhs2 of the presolves for pt >= 3 (rhsb[rhs_cur] is rhs1), the
+  // previous sub-iterate and the partial sums of its test for nsubite > 1
+  int pt = 1, nsubite = 1;
+  void *schmem = nullptr;
+  double *rhsb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  int rhs_cur = 0;
+  double *subq[3] = {nullptr, nullptr, nullptr}, *subp = nullptr;
+  double *cpart = nullptr, *csum = nullptr;
+  int subite_done = 0;
+  double conv[2] = {0.0, 0.0};
 };
 
 namespace {
@@ -534,78 +726,212 @@ int les_force(mf_ns *s, int L, hipStream_t st) {
   return MF_OK;
 }
 
-int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
+int dder(mf_ns *s, int axis, const double *x, double *out, hipStream_t st) {
+  return mf_der3d(s->dc[axis], axis, 2, x, out, s->nx, s->ny, s->nz, st);
+}
+
+// The extrapolation navier_extrapol makes in sub-iteration `sub`: the
+// order's coefficients on it(nt), it(nt-1), it(nt-2) in the first, level
+// it(1) (the current iterate) with coefficient 1 in the later ones.
+struct Extrap {
+  int lev[3];
+  double c[3];
+};
+
+Extrap extrap_of(const mf_ns *s, int order, int sub) {
+  Extrap e;
+  const int *it = s->it;
+  if (sub > 1) {
+    e.lev[0] = e.lev[1] = e.lev[2] = it[0];
+    e.c[0] = 1.0; e.c[1] = e.c[2] = 0.0;
+  } else {
+    e.lev[0] = it[NT - 1]; e.lev[1] = it[NT - 2]; e.lev[2] = it[NT - 3];
+    extrap_coefs(order, e.c);
+  }
+  return e;
+}
+
+void launch_extrap(mf_ns *s, double *const lv[NT], const Extrap &e, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(extrap_kernel, dim3(nblk(s->N)), dim3(256), 0, st, (const double *)lv[e.lev[0]],
+                     (const double *)lv[e.lev[1]], (const double *)lv[e.lev[2]], e.c[0], e.c[1],
+                     e.c[2], out, s->N, s->d);
+}
+
+// Velocity solves of navier_solve_u/v/w into q(it1): right-hand side from
+// rhs_kernel (rhs1 computed from the cached nonlinear terms ex, or read from
+// rhsb when rhs_in), the pressure gradient dp[c] when given, boundary values
+// bc[c]
+int velocity_solves(mf_ns *s, const Extrap &ev, bool rhs_in, double *const dp[3],
+                    const double *const bc[3], int precision, double tol, hipStream_t st) {
+  const int *it = s->it;
+  const int n1 = it[0], L0 = it[NT - 1], L1 = it[NT - 2], L2 = it[NT - 3];
+  const long sx = 2L * (s->ny - 2) * (s->nz - 2), sy = 2L * (s->nx - 2) * (s->nz - 2);
+  for (int c = 0; c < 3; ++c) {
+    RhsArgs r;
+    std::memset(&r, 0, sizeof(r));
+    r.q0 = s->q[c][L0]; r.q1 = s->q[c][L1]; r.q2 = s->q[c][L2];
+    r.f2 = s->fac[1]; r.f3 = s->fac[2]; r.f4 = s->tou == 3 ? s->fac[3] : 0.0;
+    if (rhs_in) {
+      r.rhs_in = s->rhsb[s->rhs_cur][c];
+    } else {
+      r.n0 = s->nl[c][ev.lev[0]]; r.n1 = s->nl[c][ev.lev[1]]; r.n2 = s->nl[c][ev.lev[2]];
+      r.c0 = ev.c[0]; r.c1 = ev.c[1]; r.c2 = ev.c[2];
+      r.force = s->has_force[c] ? s->force[c] : nullptr;
+      r.gflow = (c == 0 && s->flow) ? s->fstate : nullptr;
+      r.les = (s->les_type || s->sgs) ? s->lesf[c] : nullptr;
+    }
+    r.dp = dp ? dp[c] : nullptr;
+    r.re = s->re;
+    r.out = s->T[0];
+    r.N = s->N;
+    r.d = s->d;
+    hipLaunchKernelGGL(rhs_kernel, dim3(nblk(s->N)), dim3(256), 0, st, r);
+    NSCHK(hipGetLastError());
+    RC(mf_solver3d_solve(s->scu, s->T[0], -s->re * s->fac[0], s->q[c][n1], bc[c], bc[c] + sx,
+                         bc[c] + sx + sy, precision, tol, 0, nullptr, st));
+  }
+  s->nl_ok[n1] = false;
+  return MF_OK;
+}
+
+// testconv (navier3d.f90:366-412) on level it(1) against the previous
+// sub-iterate: relative changes of |q| and of p (mean difference removed),
+// norme2 = sqrt(sum W x^2); true when both are below 1e-9.  Synchronises.
+int testconv(mf_ns *s, bool *done, hipStream_t st) {
+  const int n1 = s->it[0];
+  const int np = s->nparts;
+  hipLaunchKernelGGL(conv1_kernel, dim3(np), dim3(256), 0, st, (const double *)s->W,
+                     (const double *)s->q[0][n1], (const double *)s->q[1][n1],
+                     (const double *)s->q[2][n1], (const double *)s->subq[0],
+                     (const double *)s->subq[1], (const double *)s->subq[2],
+                     (const double *)s->p[n1], (const double *)s->subp, s->N, s->cpart);
+  hipLaunchKernelGGL(psum_kernel, dim3(3), dim3(1024), 0, st, (const double *)s->cpart, np, s->csum);
+  hipLaunchKernelGGL(conv2_kernel, dim3(np), dim3(256), 0, st, (const double *)s->W,
+                     (const double *)s->p[n1], (const double *)s->subp, (const double *)s->csum,
+                     s->vol, s->N, s->cpart);
+  hipLaunchKernelGGL(psum_kernel, dim3(2), dim3(1024), 0, st, (const double *)s->cpart, np,
+                     s->csum + 3);
+  NSCHK(hipGetLastError());
+  double h[5];
+  NSCHK(hipMemcpyAsync(h, s->csum, sizeof(h), hipMemcpyDeviceToHost, st));
+  NSCHK(hipStreamSynchronize(st));
+  const double e1 = std::sqrt(h[1]) / std::sqrt(h[0]);
+  const double e2 = std::sqrt(h[4]) / std::sqrt(h[3]);
+  s->conv[0] = e1;
+  s->conv[1] = e2;
+  *done = e1 < 1e-9 && e2 < 1e-9;
+  return MF_OK;
+}
+
+// One pass of the sub-iteration loop (navier3d.f90:73-147, 254-255) on level
+// it(1); *done when testconv ends the loop.
+int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_t st) {
   const int *it = s->it;
   const int n1 = it[0], L0 = it[NT - 1], L1 = it[NT - 2], L2 = it[NT - 3];
   const long N = s->N;
+  const int pt = s->pt;
   const double fac1 = s->fac[0];
-  s->time += s->dt;
-  double cv[3], cp[3];
-  extrap_coefs(s->tou, cv);
-  extrap_coefs(s->top - 1, cp);
-  const int lv[3] = {L0, L1, L2};
+  double **T = s->T;
+  const Extrap ev = extrap_of(s, s->tou, sub), ep = extrap_of(s, s->top - 1, sub);
   // nonlinear terms of the levels the extrapolation reads
   for (int m = 0; m < 3; ++m)
-    if (cv[m] != 0.0) RC(nonlinear(s, lv[m], st));
-  // constant-flow-rate force from extrapol_v(u)
+    if (ev.c[m] != 0.0) RC(nonlinear(s, ev.lev[m], st));
+  // constant-flow-rate force from extrapol_v(u), updated at every presolve
   if (s->flow) {
     hipLaunchKernelGGL(wsum_kernel, dim3(s->nparts), dim3(256), 0, st, (const double *)s->W,
-                       (const double *)s->q[0][L0], (const double *)s->q[0][L1],
-                       (const double *)s->q[0][L2], cv[0], cv[1], cv[2], N, s->part, s->d);
+                       (const double *)s->q[0][ev.lev[0]], (const double *)s->q[0][ev.lev[1]],
+                       (const double *)s->q[0][ev.lev[2]], ev.c[0], ev.c[1], ev.c[2], N, s->part,
+                       s->d);
     hipLaunchKernelGGL(flow_kernel, dim3(1), dim3(1024), 0, st, (const double *)s->part, s->nparts,
                        s->vol, fac1, s->fstate);
   }
-  // eddy viscosity and -div(tau) from the latest level (navier3d.f90:68)
-  if (s->les_type || s->sgs) RC(les_force(s, L0, st));
+  // eddy viscosity and -div(tau) from the latest level (navier3d.f90:68); the
+  // level does not change between sub-iterations, so neither does the force
+  if (sub == 1 && (s->les_type || s->sgs)) RC(les_force(s, L0, st));
   s->grad_level = -1;  // T is scratch from here on
-  // boundary values of level it(1) with the tangential pressure gradient
-  {
+  // presolve: rhs2 = rhs1, rhs1 = f - extrapol_v(N), kept for pt >= 3
+  if (pt >= 3) {
+    s->rhs_cur ^= 1;
+    for (int c = 0; c < 3; ++c) {
+      RhsArgs r;
+      std::memset(&r, 0, sizeof(r));
+      r.n0 = s->nl[c][ev.lev[0]]; r.n1 = s->nl[c][ev.lev[1]]; r.n2 = s->nl[c][ev.lev[2]];
+      r.c0 = ev.c[0]; r.c1 = ev.c[1]; r.c2 = ev.c[2];
+      r.force = s->has_force[c] ? s->force[c] : nullptr;
+      r.gflow = (c == 0 && s->flow) ? s->fstate : nullptr;
+      r.les = (s->les_type || s->sgs) ? s->lesf[c] : nullptr;
+      r.rhs_out = s->rhsb[s->rhs_cur][c];
+      r.N = N;
+      r.d = s->d;
+      hipLaunchKernelGGL(rhs_kernel, dim3(nblk(N)), dim3(256), 0, st, r);
+    }
+    NSCHK(hipGetLastError());
+  }
+  const long sx = 2L * (s->ny - 2) * (s->nz - 2), sy = 2L * (s->nx - 2) * (s->nz - 2);
+  // velocity boundary values: with the tangential gradient of extrapol_p(phi)
+  // for pt = 1 (add_boundary_gradient), as given otherwise
+  const double *bc[3] = {s->bc0[0], s->bc0[1], s->bc0[2]};
+  if (pt == 1) {
     BcArgs b;
     std::memset(&b, 0, sizeof(b));
-    for (int c = 0; c < 3; ++c) { b.bc0[c] = s->bc0[c]; b.bcw[c] = s->bcw[c]; }
+    for (int c = 0; c < 3; ++c) { b.bc0[c] = s->bc0[c]; b.bcw[c] = s->bcw[c]; bc[c] = s->bcw[c]; }
     for (int m = 0; m < 3; ++m) {
-      b.c[m] = cp[m];
-      if (cp[m] != 0.0 && s->dphi_ok[lv[m]])
-        for (int a = 0; a < 3; ++a) b.dp[m][a] = s->dphi[lv[m]][a];
+      b.c[m] = ep.c[m];
+      if (ep.c[m] != 0.0 && s->dphi_ok[ep.lev[m]])
+        for (int a = 0; a < 3; ++a) b.dp[m][a] = s->dphi[ep.lev[m]][a];
     }
     b.inv_fac1 = 1.0 / fac1;
     b.d = s->d;
     hipLaunchKernelGGL(bc_kernel, dim3(nblk(3 * s->nbc)), dim3(256), 0, st, b);
   }
-  // velocity solves
-  const long sx = 2L * (s->ny - 2) * (s->nz - 2), sy = 2L * (s->nx - 2) * (s->nz - 2);
-  for (int c = 0; c < 3; ++c) {
-    RhsArgs r;
-    r.q0 = s->q[c][L0]; r.q1 = s->q[c][L1]; r.q2 = s->q[c][L2];
-    r.n0 = s->nl[c][L0]; r.n1 = s->nl[c][L1]; r.n2 = s->nl[c][L2];
-    r.f2 = s->fac[1]; r.f3 = s->fac[2]; r.f4 = s->tou == 3 ? s->fac[3] : 0.0;
-    r.c0 = cv[0]; r.c1 = cv[1]; r.c2 = cv[2];
-    r.force = s->has_force[c] ? s->force[c] : nullptr;
-    r.gflow = (c == 0 && s->flow) ? s->fstate : nullptr;
-    r.les = (s->les_type || s->sgs) ? s->lesf[c] : nullptr;
-    r.re = s->re;
-    r.out = s->T[0];
+  // predicted velocity (pt <= 2), with d_c extrapol_p(p) for pt = 2
+  if (pt <= 2) {
+    double *dp[3] = {T[21], T[22], T[23]};
+    if (pt == 2) {
+      launch_extrap(s, s->p, ep, T[20], st);
+      for (int a = 0; a < 3; ++a) RC(der(s, a, T[20], dp[a], st));
+    }
+    RC(velocity_solves(s, ev, false, pt == 2 ? dp : nullptr, bc, precision, tol, st));
+  }
+  // pressure: rhs_p, fphi = div rhs_p, Neumann values, solve
+  {
+    RpArgs r;
+    std::memset(&r, 0, sizeof(r));
+    for (int c = 0; c < 3; ++c) {
+      r.qi[c] = s->q[c][n1]; r.q0[c] = s->q[c][L0]; r.q1[c] = s->q[c][L1]; r.q2[c] = s->q[c][L2];
+      if (pt >= 3) { r.r1[c] = s->rhsb[s->rhs_cur][c]; r.r2[c] = s->rhsb[s->rhs_cur ^ 1][c]; }
+      r.out[c] = T[c];
+    }
+    r.f1 = s->fac[0]; r.f2 = s->fac[1]; r.f3 = s->fac[2]; r.f4 = s->fac[3];
+    r.mode = pt <= 2 ? 0 : (pt == 3 ? 3 : (sub == 1 ? 4 : 5));
     r.N = N;
     r.d = s->d;
-    hipLaunchKernelGGL(rhs_kernel, dim3(nblk(N)), dim3(256), 0, st, r);
-    NSCHK(hipGetLastError());
-    const double *bc = s->bcw[c];
-    RC(mf_solver3d_solve(s->scu, s->T[0], -s->re * fac1, s->q[c][n1], bc, bc + sx, bc + sx + sy,
-                         precision, tol, 0, nullptr, st));
+    hipLaunchKernelGGL(rp_kernel, dim3(nblk(N)), dim3(256), 0, st, r);
   }
-  // pressure: rhs_p = fac1 u*, fphi = div rhs_p, Neumann values, solve
-  double **T = s->T;
-  hipLaunchKernelGGL(rp_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)s->q[0][n1],
-                     (const double *)s->q[1][n1], (const double *)s->q[2][n1], fac1, T[0], T[1],
-                     T[2], N, s->d);
   for (int a = 0; a < 3; ++a) RC(der(s, a, T[a], T[3 + a], st));
   hipLaunchKernelGGL(sum3_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[3],
                      (const double *)T[4], (const double *)T[5], T[6], N);
   hipLaunchKernelGGL(bcphi_kernel, dim3(nblk(s->nbc)), dim3(256), 0, st, (const double *)T[0],
-                     (const double *)T[1], (const double *)T[2], (const double *)s->bcw[0],
-                     (const double *)(s->bcw[1] + sx), (const double *)(s->bcw[2] + sx + sy), fac1,
-                     s->bcphi, s->d);
+                     (const double *)T[1], (const double *)T[2], bc[0], bc[1] + sx, bc[2] + sx + sy,
+                     fac1, s->bcphi, s->d);
   NSCHK(hipGetLastError());
+  if (pt == 3) {  // add_boundary_rotrot on extrapol_p of the velocity
+    double *e[3] = {T[7], T[8], T[9]}, *de[3] = {T[10], T[11], T[12]};
+    for (int c = 0; c < 3; ++c) launch_extrap(s, s->q[c], ep, e[c], st);
+    for (int c = 0; c < 3; ++c) RC(der(s, c, e[c], de[c], st));  // d_c e_c
+    for (int a = 0; a < 3; ++a) {
+      const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+      hipLaunchKernelGGL(sum2_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)de[b],
+                         (const double *)de[c], T[13], N);
+      RC(der(s, a, T[13], T[14], st));
+      RC(dder(s, b, e[a], T[15], st));
+      RC(dder(s, c, e[a], T[16], st));
+      const long nf = a == 0 ? sx : (a == 1 ? sy : 2L * (s->nx - 2) * (s->ny - 2));
+      hipLaunchKernelGGL(rotrot_kernel, dim3(nblk(nf)), dim3(256), 0, st, a, (const double *)T[14],
+                         (const double *)T[15], (const double *)T[16], s->re, s->bcphi, s->d);
+    }
+    NSCHK(hipGetLastError());
+  }
   RC(mf_solver3d_solve(s->scp, T[6], 0.0, s->phi[n1], s->bcphi, s->bcphi + sx, s->bcphi + sx + sy,
                        MF_FP64, 0.0, 0, nullptr, st));
   // projection
@@ -613,13 +939,62 @@ int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
   hipLaunchKernelGGL(zero_edges_kernel, dim3(nblk(ne)), dim3(256), 0, st, s->phi[n1], s->d);
   for (int a = 0; a < 3; ++a) RC(der(s, a, s->phi[n1], s->dphi[n1][a], st));
   s->dphi_ok[n1] = true;
-  hipLaunchKernelGGL(proj_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)s->phi[n1],
-                     (const double *)T[6], (const double *)T[0], (const double *)T[1],
-                     (const double *)T[2], (const double *)s->dphi[n1][0],
-                     (const double *)s->dphi[n1][1], (const double *)s->dphi[n1][2], fac1, s->re,
-                     s->p[n1], s->q[0][n1], s->q[1][n1], s->q[2][n1], N, s->d);
-  NSCHK(hipGetLastError());
+  {
+    ProjArgs pj;
+    std::memset(&pj, 0, sizeof(pj));
+    pj.phi = s->phi[n1];
+    if (pt == 2 || pt == 4) {
+      launch_extrap(s, s->p, ep, T[7], st);
+      pj.pe = T[7];
+    }
+    if (pt <= 2) {
+      pj.dv = T[6];  // div(fac1 u*)
+      pj.dv_fac1 = 1;
+    } else if (pt == 4) {  // div(extrapol_p(q))
+      for (int c = 0; c < 3; ++c) {
+        launch_extrap(s, s->q[c], ep, T[8 + c], st);
+        RC(der(s, c, T[8 + c], T[11 + c], st));
+      }
+      hipLaunchKernelGGL(sum3_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[11],
+                         (const double *)T[12], (const double *)T[13], T[14], N);
+      pj.dv = T[14];
+    }
+    for (int c = 0; c < 3; ++c) { pj.r[c] = T[c]; pj.g[c] = s->dphi[n1][c]; pj.q[c] = s->q[c][n1]; }
+    pj.fac1 = fac1;
+    pj.re = s->re;
+    pj.p = s->p[n1];
+    pj.N = N;
+    pj.d = s->d;
+    hipLaunchKernelGGL(proj_kernel, dim3(nblk(N)), dim3(256), 0, st, pj);
+    NSCHK(hipGetLastError());
+  }
   s->nl_ok[n1] = false;
+  // velocity correction (pt >= 3) with the gradient of the new pressure
+  if (pt >= 3) {
+    double *dp[3] = {T[21], T[22], T[23]};
+    for (int a = 0; a < 3; ++a) RC(der(s, a, s->p[n1], dp[a], st));
+    RC(velocity_solves(s, ev, true, dp, bc, precision, tol, st));
+  }
+  *done = false;
+  if (sub > 1) RC(testconv(s, done, st));
+  if (s->nsubite > 1) {
+    const size_t b = N * sizeof(double);
+    for (int c = 0; c < 3; ++c)
+      NSCHK(hipMemcpyAsync(s->subq[c], s->q[c][n1], b, hipMemcpyDeviceToDevice, st));
+    NSCHK(hipMemcpyAsync(s->subp, s->p[n1], b, hipMemcpyDeviceToDevice, st));
+  }
+  return MF_OK;
+}
+
+int one_step(mf_ns *s, int precision, double tol, hipStream_t st) {
+  s->time += s->dt;
+  s->subite_done = 0;
+  for (int sub = 1; sub <= s->nsubite; ++sub) {
+    bool done = false;
+    RC(substep(s, sub, precision, tol, &done, st));
+    s->subite_done = sub;
+    if (done) break;
+  }
   // rotate the time levels
   const int first = s->it[0];
   for (int m = 0; m < NT - 1; ++m) s->it[m] = s->it[m + 1];
@@ -636,6 +1011,7 @@ extern "C" int mf_ns_destroy(mf_ns *s) {
   if (s->mem) (void)hipFree(s->mem);
   if (s->lesmem) (void)hipFree(s->lesmem);
   if (s->sgsmem) (void)hipFree(s->sgsmem);
+  if (s->schmem) (void)hipFree(s->schmem);
   delete s;
   return MF_OK;
 }
@@ -798,6 +1174,52 @@ extern "C" int mf_ns_set_les(mf_ns *s, int les_type, double c, void *stream) {
   return MF_OK;
 }
 
+extern "C" int mf_ns_set_scheme(mf_ns *s, int pt, int nsubite, void *stream) {
+  if (!s || pt < 1 || pt > 4 || nsubite < 1)
+    return mf::fail(MF_ERR_ARG, "mf_ns_set_scheme: pt must be 1..4 and nsubite >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  NSCHK(hipStreamSynchronize(st));
+  if (s->schmem) {
+    NSCHK(hipFree(s->schmem));
+    s->schmem = nullptr;
+  }
+  for (int k = 0; k < 2; ++k)
+    for (int c = 0; c < 3; ++c) s->rhsb[k][c] = nullptr;
+  for (int c = 0; c < 3; ++c) s->subq[c] = nullptr;
+  s->subp = s->cpart = s->csum = nullptr;
+  const long Na = (s->N + 31) / 32 * 32;
+  const long nrhs = pt >= 3 ? 6 : 0, nsub = nsubite > 1 ? 4 : 0;
+  const size_t total = (size_t)(nrhs + nsub) * Na + (nsub ? 3L * s->nparts + 8 : 0);
+  if (total) {
+    NSCHK(hipMalloc(&s->schmem, total * sizeof(double)));
+    NSCHK(hipMemsetAsync(s->schmem, 0, total * sizeof(double), st));
+    double *m = (double *)s->schmem;
+    for (int k = 0; k < 2 && nrhs; ++k)
+      for (int c = 0; c < 3; ++c, m += Na) s->rhsb[k][c] = m;
+    if (nsub) {
+      for (int c = 0; c < 3; ++c, m += Na) s->subq[c] = m;
+      s->subp = m; m += Na;
+      s->cpart = m; m += 3L * s->nparts;
+      s->csum = m;
+    }
+    NSCHK(hipStreamSynchronize(st));
+  }
+  s->pt = pt;
+  s->nsubite = nsubite;
+  s->rhs_cur = 0;
+  s->subite_done = 0;
+  s->conv[0] = s->conv[1] = 0.0;
+  return MF_OK;
+}
+
+extern "C" int mf_ns_subiterations(const mf_ns *s, int *count, double *dvel, double *dp) {
+  if (!s) return mf::fail(MF_ERR_ARG, "mf_ns_subiterations: null handle");
+  if (count) *count = s->subite_done;
+  if (dvel) *dvel = s->conv[0];
+  if (dp) *dp = s->conv[1];
+  return MF_OK;
+}
+
 extern "C" int mf_ns_set_sgs(mf_ns *s, const mf_mlp *mlp, double gscale, double lscale,
                              double tscale, void *stream) {
   if (!s) return mf::fail(MF_ERR_ARG, "mf_ns_set_sgs: null handle");
@@ -817,6 +1239,18 @@ extern "C" int mf_ns_set_sgs(mf_ns *s, const mf_mlp *mlp, double gscale, double 
   (void)stream;
   return MF_OK;
 }
+
+namespace {
+// rhs1 = rhs2 = 0, as after navier_initialization (field_init zeroes them,
+// class_navier_3D.f90:2240-2244; the restart files do not hold them)
+int clear_presolve(mf_ns *s, hipStream_t st) {
+  for (int k = 0; k < 2; ++k)
+    for (int c = 0; c < 3; ++c)
+      if (s->rhsb[k][c]) NSCHK(hipMemsetAsync(s->rhsb[k][c], 0, s->N * sizeof(double), st));
+  s->rhs_cur = 0;
+  return MF_OK;
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Restart files (restart_write / restart_read, class_navier_3D.f90:186-340)
@@ -948,6 +1382,7 @@ extern "C" int mf_ns_restart_read(mf_ns *s, const char *dirname, void *stream) {
     s->dphi_ok[L] = true;
   }
   s->grad_level = -1;
+  RC(clear_presolve(s, st));
   const double zero = 0.0;
   NSCHK(hipMemcpyAsync(s->fstate + 2, &zero, sizeof(double), hipMemcpyHostToDevice, st));
   NSCHK(hipStreamSynchronize(st));
@@ -982,6 +1417,7 @@ extern "C" int mf_ns_set_state(mf_ns *s, const double *u, const double *v, const
   for (int m = 0; m < NT; ++m) s->it[m] = m;
   s->time = 0.0;
   s->grad_level = -1;
+  RC(clear_presolve(s, st));
   NSCHK(hipGetLastError());
   return MF_OK;
 }

@@ -15,7 +15,7 @@
 !     read_var3d(file_name,var_name,var)                           :126-189
 !     get_var3d_info(file_name,var_name,dim_name,dim_len)          :191-223
 !   module mflops_navier       <- class_navier_3D     src/class_navier/class_navier_3D.f90
-!     navier_initialization(nav,gx,gy,gz,re,dt[,tou,top,nlt,so])    :2084-2345
+!     navier_initialization(nav,gx,gy,gz,re,dt[,tou,top,nlt,so,pt,nsubite]) :2084-2345
 !     navier_set_bc / navier_set_state / navier_set_flow_rate / navier_set_les,
 !     navier_step(nav,nsteps[,precision]) (navier3d.f90:48-286), navier_get,
 !     restart_write(nav[,dirname]) :186-257, restart_read(nav[,dirname]) :259-340
@@ -515,6 +515,12 @@ module mflops_navier
        real(c_double), value :: c
        integer(c_int) :: rc
      end function mf_ns_set_les
+     function mf_ns_set_scheme(h, pt, nsubite, st) bind(c, name="mf_ns_set_scheme") result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, st
+       integer(c_int), value :: pt, nsubite
+       integer(c_int) :: rc
+     end function mf_ns_set_scheme
      function mf_ns_step(h, nsteps, precision, tol, st) bind(c, name="mf_ns_step") result(rc)
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: h, st
@@ -560,11 +566,13 @@ contains
   end subroutine hip_check
 
   ! navier_initialization: grids (full), Re, dt, BDF order tou, pressure
-  ! extrapolation top, nonlinear form nlt, scheme order so (defaults 2, 2, 2, 6)
-  subroutine navier_initialization(nav, gx, gy, gz, re, dt, tou, top, nlt, so)
+  ! extrapolation top, nonlinear form nlt, scheme order so, projection type
+  ! pt and sub-iterations nsubite (defaults 2, 2, 2, 6, 1, 1)
+  subroutine navier_initialization(nav, gx, gy, gz, re, dt, tou, top, nlt, so, pt, nsubite)
     type(navier3d), intent(inout) :: nav
     real(c_double), intent(in) :: gx(:), gy(:), gz(:), re, dt
-    integer, intent(in), optional :: tou, top, nlt, so
+    integer, intent(in), optional :: tou, top, nlt, so, pt, nsubite
+    integer(c_int) :: p1, p2
     integer(c_int) :: t1, t2, t3, t4
     integer :: q
     integer(c_size_t) :: bytes
@@ -577,6 +585,11 @@ contains
     nav%nx = size(gx); nav%ny = size(gy); nav%nz = size(gz)
     call mf_check(mf_ns_create(gx, int(nav%nx, c_int), gy, int(nav%ny, c_int), gz, int(nav%nz, c_int), &
          re, dt, t1, t2, t3, t4, nav%h), 'navier_initialization')
+    p1 = 1; p2 = 1
+    if (present(pt)) p1 = int(pt, c_int)
+    if (present(nsubite)) p2 = int(nsubite, c_int)
+    if (p1 /= 1 .or. p2 /= 1) &
+         call mf_check(mf_ns_set_scheme(nav%h, p1, p2, c_null_ptr), 'navier_initialization')
     bytes = int(nav%nx, c_size_t) * nav%ny * nav%nz * 8
     do q = 1, 5
        call hip_check(hipMalloc(nav%stage(q), bytes), 'navier_initialization')

@@ -370,12 +370,13 @@ def split_faces(row: np.ndarray, shape):
 
 class NavierStokes:
     """navier3d state on the device (class_navier_3D.f90:26-76) with the
-    reference's pt=1 step; see mf_ns_* in include/mflops/mf.h.
+    reference's step: projection type pt (1..4, default 1) and up to nsubite
+    sub-iterations per step (default 1); see mf_ns_* in include/mflops/mf.h.
 
     Fields are full (nx, ny, nz) Fortran-order float64 device tensors."""
 
     def __init__(self, gx, gy, gz, re: float, dt: float, tou: int = 2, top: int = 2,
-                 nlt: int = 2, so: int = 6):
+                 nlt: int = 2, so: int = 6, pt: int = 1, nsubite: int = 1):
         lib = load()
         self._lib = lib
         gs = [np.ascontiguousarray(g, dtype=np.float64) for g in (gx, gy, gz)]
@@ -386,6 +387,20 @@ class NavierStokes:
                                _dptr(gs[2]), len(gs[2]), float(re), float(dt), int(tou),
                                int(top), int(nlt), int(so), C.byref(h)))
         self._h = h
+        if pt != 1 or nsubite != 1:
+            self.set_scheme(pt, nsubite)
+
+    def set_scheme(self, pt: int, nsubite: int = 1, stream=None):
+        """Projection type pt (1..4) and the maximum number of sub-iterations
+        per step; clears the stored presolve right-hand sides."""
+        check(self._lib.mf_ns_set_scheme(self._h, int(pt), int(nsubite), _stream(stream)))
+
+    def subiterations(self):
+        """(count, velocity change, pressure change) of the last step's
+        sub-iterations (the changes of its last convergence test)."""
+        n, dv, dp = C.c_int(), C.c_double(), C.c_double()
+        check(self._lib.mf_ns_subiterations(self._h, C.byref(n), C.byref(dv), C.byref(dp)))
+        return n.value, dv.value, dp.value
 
     def __del__(self):
         h = getattr(self, "_h", None)

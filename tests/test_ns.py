@@ -140,7 +140,7 @@ def test_laminar_channel_is_steady(flow):
         assert abs(s.flow.g - 2.0 / re) < 1e-9
 
 
-def _mms(n, sign, re=10.0, dt=1e-2, steps=30):
+def _mms(n, sign, re=10.0, dt=1e-2, steps=30, pt=1):
     x, y, z = sp.symbols("x y z")
     X = (x, y, z)
     psi = sp.sin(sp.pi * x) ** 2 * sp.sin(sp.pi * y) ** 2 * sp.cos(sp.pi * z / 2) / sp.pi
@@ -156,7 +156,7 @@ def _mms(n, sign, re=10.0, dt=1e-2, steps=30):
     ev = lambda e: np.asfortranarray(np.broadcast_to(
         sp.lambdify(X, e, "numpy")(XX, YY, ZZ), XX.shape).astype(float))
     u = [ev(e) for e in U]
-    s = ns.NSOracle(g, g, g, re, dt)
+    s = ns.NSOracle(g, g, g, re, dt, pt=pt)
     s.set_state(*u)
     for c in range(3):
         s.force["uvw"[c]] = ev(F[c])
@@ -177,6 +177,107 @@ def test_manufactured_steady_flow():
     flipped = _mms(16, -1)
     assert e16 < 0.02 and e24 < 0.6 * e16, (e16, e24)
     assert flipped > 0.1, flipped
+
+
+def _face_bc(q):
+    return (np.asfortranarray(np.stack([q[0, 1:-1, 1:-1], q[-1, 1:-1, 1:-1]], -1)),
+            np.asfortranarray(np.stack([q[1:-1, 0, 1:-1], q[1:-1, -1, 1:-1]], -1)),
+            np.asfortranarray(np.stack([q[1:-1, 1:-1, 0], q[1:-1, 1:-1, -1]], -1)))
+
+
+def _unsteady_mms(n, pt, dt, T, nsubite=1, top=2, cls=None, re=10.0):
+    """u = (psi_y, -psi_x, 0) (1 + 4 t^3), p = P 4 t^3 on [0, 1]^3, forced so
+    that they solve the Navier-Stokes equations; the forcing and the boundary
+    values are set at t^{n+1} before each step (f(nav,'rhsu') and
+    navier_bc_velocity read the already advanced time).  Both time factors
+    are flat at t = 0, so the constant start history and p = 0 are exact to
+    O(dt^3).  Returns the oracle after T / dt steps and the exact velocity."""
+    x, y, z, t = sp.symbols("x y z t")
+    X = (x, y, z)
+    psi = sp.sin(sp.pi * x) ** 2 * sp.sin(sp.pi * y) ** 2 * sp.cos(sp.pi * z / 2) / sp.pi
+    U = [sp.diff(psi, y) * (1 + 4 * t ** 3), -sp.diff(psi, x) * (1 + 4 * t ** 3), sp.Integer(0)]
+    P = sp.cos(sp.pi * x) * sp.cos(sp.pi * y) * sp.sin(sp.pi * z) * 4 * t ** 3
+    F = [sp.diff(U[c], t) + sum(U[a] * sp.diff(U[c], X[a]) for a in range(3)) + sp.diff(P, X[c])
+         - sum(sp.diff(U[c], X[a], 2) for a in range(3)) / re for c in range(3)]
+    g = o.grid_uniform(n, 0.0, 1.0)
+    XX, YY, ZZ = np.meshgrid(g, g, g, indexing="ij")
+    fU = [sp.lambdify((x, y, z, t), e, "numpy") for e in U]
+    fF = [sp.lambdify((x, y, z, t), e, "numpy") for e in F]
+    ev = lambda f, tt: np.asfortranarray(np.broadcast_to(f(XX, YY, ZZ, tt), XX.shape).astype(float))
+    s = (cls or ns.NSOracle)(g, g, g, re, dt, pt=pt, nsubite=nsubite, top=top)
+    s.set_state(*[ev(f, 0.0) for f in fU])
+    steps = int(round(T / dt))
+    for k in range(steps):
+        tn = (k + 1) * dt
+        for c in range(3):
+            s.force["uvw"[c]] = ev(fF[c], tn)
+            s.bc["uvw"[c]] = _face_bc(ev(fU[c], tn))
+        s.step()
+    return s, [ev(f, steps * dt) for f in fU]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("pt", [2, 3, 4])
+def test_incremental_schemes_keep_the_steady_flow(pt):
+    """pt = 2, 3, 4 carry the pressure from step to step, so the manufactured
+    steady flow is kept to the spatial error, which falls with the grid, far
+    below the non-incremental scheme's splitting error (7.7e-3 at 16^3)."""
+    e16, e24 = _mms(16, 1, pt=pt), _mms(24, 1, pt=pt)
+    assert e16 < 1e-3 and e24 < 0.3 * e16, (e16, e24)
+
+
+class _RotRotFlipped(ns.NSOracle):
+    """The oracle with a planted mistake: the rot-rot boundary term of pt=3
+    added instead of subtracted."""
+
+    def add_boundary_rotrot(self, bcphi, sub):
+        before = [b.copy() for b in bcphi]
+        super().add_boundary_rotrot(bcphi, sub)
+        for a in range(3):
+            bcphi[a][...] = 2.0 * before[a] - bcphi[a]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("pt", [1, 2, 3, 4])
+def test_unsteady_manufactured_flow_converges_in_time(pt):
+    """Every projection type follows an unsteady exact solution: at 12^3 the
+    error at t = 0.4 is at the spatial floor (~2e-3) for dt = 0.01, and the
+    differences between dt = 0.04, 0.02, 0.01 shrink by more than 2 per
+    halving.  A wrong increment (the rot-rot term's sign flipped) is off by
+    more than 1e-2 (measured 2.9e-2)."""
+    runs = [_unsteady_mms(12, pt, dt, 0.4) for dt in (0.04, 0.02, 0.01)]
+    v = [np.stack(s.velocity()) for s, _ in runs]
+    exact = np.stack(runs[-1][1])
+    err = np.abs(v[-1] - exact).max() / np.abs(exact).max()
+    ratio = np.abs(v[0] - v[1]).max() / np.abs(v[1] - v[2]).max()
+    assert err < 3e-3 and ratio > 2.0, (err, ratio)
+    if pt == 3:
+        s, ex = _unsteady_mms(12, 3, 0.01, 0.4, cls=_RotRotFlipped)
+        bad = np.abs(np.stack(s.velocity()) - np.stack(ex)).max() / np.abs(exact).max()
+        assert bad > 1e-2, bad
+
+
+@pytest.mark.parametrize("pt", [1, 3])
+def test_subiterations_reach_a_fixed_point_independent_of_top(pt):
+    """From the second sub-iteration on every extrapolation returns the
+    current iterate, so converged sub-iterations solve a problem that no
+    longer depends on the pressure extrapolation order top: top = 1, 2, 3
+    agree to the 1e-9 stopping test, while single sub-iterations differ by
+    more than 1e-4.  (pt = 2 and 4 accumulate phi - div(u*)/Re into p at
+    every sub-iteration, which does not vanish, so their sub-iterations run
+    to nsubite.)"""
+    def run(top, nsub):
+        s, _ = _unsteady_mms(12, pt, 0.02, 0.04, nsubite=nsub, top=top)
+        W = s.W
+        p = s.pressure() - (W * s.pressure()).sum() / W.sum()
+        return np.stack(s.velocity()), p, s.subite_done
+    one = [run(top, 1) for top in (1, 2, 3)]
+    many = [run(top, 40) for top in (1, 2, 3)]
+    spread = lambda r, k: max(np.abs(r[0][k] - r[m][k])[1:-1, 1:-1, 1:-1].max() for m in (1, 2)) \
+        / np.abs(r[0][k]).max()
+    assert spread(one, 0) > 1e-4
+    assert spread(many, 0) < 1e-10 and spread(many, 1) < 1e-8, (spread(many, 0), spread(many, 1))
+    assert all(1 < r[2] < 40 for r in many)
 
 
 def test_projection_reduces_divergence():
@@ -256,6 +357,121 @@ def test_ns_steps_match_oracle(tou, top):
     t, g = dev.info()
     assert abs(t - steps * dt) < 1e-12
     assert abs(g - orc.flow.g) < 1e-10 * abs(orc.flow.g)
+
+
+def _orc_fields(orc):
+    return list(orc.velocity()) + [orc.pressure(), orc.phi_latest()]
+
+
+def _rel_errs(got, want):
+    return {name: np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+            for name, a, b in zip(("u", "v", "w", "p", "phi"), got, want)}
+
+
+def _compare(mf, dev, orc, tol, label):
+    """Fields of dev against the oracle, relative to the oracle's maximum;
+    tol is a number or a dict per field."""
+    errs = _rel_errs([mf.to_host(t) for t in dev.get()], _orc_fields(orc))
+    print("  %s: %s" % (label, {k: "%.2e" % v for k, v in errs.items()}), flush=True)
+    bar = tol if isinstance(tol, dict) else {k: tol for k in errs}
+    assert all(errs[k] < bar[k] for k in errs), (errs, bar)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pt,tou,top", [(2, 2, 2), (3, 2, 2), (4, 2, 2), (3, 3, 3), (4, 3, 3),
+                                        (2, 1, 1)])
+def test_ns_projection_types_match_oracle(pt, tou, top):
+    """Projection types 2 (incremental), 3 (velocity correction with the
+    rot-rot pressure boundary term) and 4 (incremental velocity correction)
+    over 5 steps of the perturbed channel with the flow-rate force: every
+    field within 1e-10 of the oracle (relative to its maximum)."""
+    import mflops as mf
+    import torch
+    assert torch.cuda.is_available()
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((20, 24, 18))
+    re, dt, steps = 3000.0, 1e-3, 5
+    print("run ns pt=%d tou=%d top=%d shape=%s steps=%d" % (pt, tou, top, u.shape, steps), flush=True)
+    orc = ns.NSOracle(gx, gy, gz, re, dt, tou=tou, top=top, pt=pt)
+    orc.set_state(u, v, w)
+    for c in range(3):
+        orc.bc["uvw"[c]] = bc[c]
+    orc.flow = ns.FlowRate(g0=2.0 / re)
+    for _ in range(steps):
+        orc.step()
+    dev = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt, tou=tou, top=top, pt=pt)
+    dev.set_flow_rate(True, 2.0 / re)
+    dev.step(steps, mf.MF_FP64)
+    _compare(mf, dev, orc, 1e-10, "pt=%d tou=%d top=%d" % (pt, tou, top))
+    t, g = dev.info()
+    assert abs(g - orc.flow.g) < 1e-10 * abs(orc.flow.g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pt,nsub,les", [(1, 20, 0), (3, 20, 0), (2, 4, 0), (4, 4, 1), (3, 20, 1)])
+def test_ns_subiterations_match_oracle(pt, nsub, les):
+    """Sub-iterations (nsubite) on the channel with a uniform body force:
+    pt = 1 and 3 stop on the convergence test (after 7 and 5 sub-iterations
+    in the oracle), pt = 2 and 4 run to nsubite; 3 steps, the same number of
+    sub-iterations, the last test's changes alike (to 1e-6 where they are far
+    above rounding), fields within 1e-10 of the oracle.  The sub-iterations
+    of pt = 2 and 4 do not contract (each moves p by 20-50%), so they amplify
+    rounding: there the bar of each field is the larger of 1e-10 and how far
+    the oracle itself moves when every velocity solve's result is perturbed
+    by 1e-13 (relative, random)."""
+    import mflops as mf
+    import torch
+    assert torch.cuda.is_available()
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((20, 24, 18))
+    re, dt, steps = 3000.0, 1e-3, 3
+    print("run ns subiterations pt=%d nsubite=%d les=%d steps=%d" % (pt, nsub, les, steps), flush=True)
+    force = np.asfortranarray(np.full(u.shape, 2.0 / re))
+    orc = ns.NSOracle(gx, gy, gz, re, dt, pt=pt, nsubite=nsub)
+    orc.set_state(u, v, w)
+    for c in range(3):
+        orc.bc["uvw"[c]] = bc[c]
+    orc.force["u"] = force
+    if les:
+        orc.set_les(1, 0.165)
+    dev = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt, pt=pt, nsubite=nsub)
+    dev.set_forcing(0, mf.to_device(force))
+    if les:
+        dev.set_les(1, 0.165)
+    for k in range(steps):
+        orc.step()
+        dev.step(1, mf.MF_FP64)
+        n, dv, dp = dev.subiterations()
+        print("  step %d: sub-iterations gpu %d oracle %d, changes gpu (%.2e, %.2e) oracle (%.2e, %.2e)"
+              % (k, n, orc.subite_done, dv, dp, *orc.conv), flush=True)
+        assert n == orc.subite_done
+        for a, b in zip((dv, dp), orc.conv):
+            if b > 1e-6:
+                assert abs(a - b) <= 1e-6 * b, (a, b)
+            else:
+                assert a < 1e-9 and b < 1e-9 or n == nsub
+    bar = 1e-10
+    if pt in (2, 4):
+        pert = ns.NSOracle(gx, gy, gz, re, dt, pt=pt, nsubite=nsub)
+        pert.set_state(u, v, w)
+        for c in range(3):
+            pert.bc["uvw"[c]] = bc[c]
+        pert.force["u"] = force
+        if les:
+            pert.set_les(1, 0.165)
+        rng = np.random.default_rng(1)
+
+        def hook(q):
+            q *= 1.0 + 1e-13 * rng.standard_normal(q.shape)
+
+        pert.velocity_hook = hook
+        for _ in range(steps):
+            pert.step()
+        sens = _rel_errs(_orc_fields(pert), _orc_fields(orc))
+        print("  oracle moved by 1e-13 perturbations: %s" % {k: "%.2e" % v for k, v in sens.items()},
+              flush=True)
+        bar = {k: max(1e-10, v) for k, v in sens.items()}
+    _compare(mf, dev, orc, bar, "pt=%d nsubite=%d" % (pt, nsub))
 
 
 @pytest.mark.gpu
