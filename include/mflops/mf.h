@@ -214,6 +214,27 @@ int mf_md_set_guess(mf_md *h, int enable);
 int mf_md_write_matrix(const mf_md *h, const char *path);
 int mf_md_create_cached(const int nd[3], const int *npts, const double *grids, const int bct[6],
                         int so, double sigma, const char *path, int *loaded, mf_md **out);
+/* <- mpid%periods (md_mpi_init, class_md.f90) as used by
+ *    md_influence_matrix_param (class_md.f90:1482-1560): mf_md_create with
+ *    periodic axes (periods[a] != 0; NULL = none).  Along a periodic axis the
+ *    last subdomain's high face and the first one's low face are one more
+ *    interface (numbered where the last subdomain's interface of that axis
+ *    stands; one subdomain along the axis: its two faces carry the same
+ *    interface); the grids of such an axis run over one period.  With path
+ *    non-NULL, as mf_md_create_cached (the cache records the periods). */
+int mf_md_create_ex(const int nd[3], const int *npts, const double *grids, const int bct[6],
+                    const int periods[3], int so, double sigma, const char *path, int *loaded,
+                    mf_md **out);
+/* <- psm (command_line.f90:27-29; navier_initialization :2119-2121), passed
+ *    as null= to md_solve_init (class_md.f90:318-328) and multidomain_solve
+ *    (class_field.f90:1338-1352) for singular interface systems (sigma = 0,
+ *    outer faces Neumann or periodic): 0 solve as is; 1 the constant is the
+ *    null space: removed from the right-hand side and from every Krylov
+ *    vector, so x keeps the mean of its start (zero without the guess
+ *    history) and M x = rhs - c 1; 2 the first right-hand-side entry is set
+ *    to zero (md_vector_zero_lastpoint :251-267; the reference does so on the
+ *    pressure solve, the one caller that passes null).  Default 0. */
+int mf_md_set_null_space(mf_md *h, int psm);
 
 /* Kernel timing (no counterpart in the reference; its solve_3d has
  * commented-out cpu_time probes, class_solver_3d.f90:106-259).  While

@@ -21,7 +21,13 @@
 // twice, fixed-order dot products; the small Hessenberg least-squares
 // problem on the host).  Interface numbering and unknown order follow
 // md_influence_matrix_param (class_md.f90:1450-1630), generalised to one
-// point count per slab.
+// point count per slab; a periodic axis adds the interface that joins the
+// last subdomain's high face to the first one's low face (:1482-1560).
+// Singular systems (sigma = 0 with only Neumann or periodic outer faces: the
+// constant interface vector is a null vector of M) follow the pressure
+// singular method psm: 1 removes the constant from the right-hand side and
+// from every Krylov vector, as PETSc's constant null space does, so x keeps
+// the mean of its start; 2 zeroes the first right-hand-side entry.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -240,9 +246,6 @@ __global__ void __launch_bounds__(256) bc_iface_kernel(double *__restrict__ bc,
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
-// set while mf_md_create_cached builds a handle whose blocks it reads from a file
-thread_local bool skip_assembly = false;
-
 }  // namespace md
 }  // namespace mf
 
@@ -265,7 +268,13 @@ struct mf_md {
   };
   int nd[3] = {0, 0, 0};
   int bct[6];
+  int periods[3] = {0, 0, 0};
   double sigma = 0.0;
+  // pressure singular method (command_line.f90:27-29): 0 none, 1 constant
+  // null space (md_rhs_nullspace + KSPSetNullSpace), 2 first right-hand-side
+  // entry zeroed (md_vector_zero_lastpoint)
+  int psm = 0;
+  double *ones = nullptr;  // ninf ones, for psm = 1
   std::vector<Sub> subs;
   std::vector<long> ioff;  // interface offsets (size it+1)
   long ninf = 0, nloc = 0;
@@ -300,6 +309,7 @@ struct mf_md {
     if (pb) (void)hipFree(pb);
     if (ws) (void)hipFree(ws);
     if (hist) (void)hipFree(hist);
+    if (ones) (void)hipFree(ones);
   }
 };
 
@@ -383,6 +393,16 @@ int dots(mf_md *h, const Ws &w, const double *V, int k, const double *v, double 
   return MF_OK;
 }
 
+// v -= mean(v) (the constant null space of psm = 1)
+int remove_constant(mf_md *h, const Ws &w, double *v, hipStream_t st) {
+  double sum = 0.0;
+  MDRC(dots(h, w, h->ones, 1, v, &sum, st));
+  hipLaunchKernelGGL(mf::md::axpby_kernel, dim3(mf::md::nblk(h->ninf)), dim3(256), 0, st,
+                     -sum / (double)h->ninf, (const double *)h->ones, 1.0, v, h->ninf);
+  MDCHK(hipGetLastError());
+  return MF_OK;
+}
+
 // Solve all subdomains with their boundary arrays (interface faces from x,
 // or zero when x is null) into u[s].
 int solve_subs(mf_md *h, const double *const *g, double *const *u, const double *const *bc,
@@ -411,8 +431,9 @@ int solve_subs(mf_md *h, const double *const *g, double *const *u, const double 
 
 }  // namespace
 
-extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grids, const int bct[6],
-                            int so, double sigma, mf_md **out) {
+namespace {
+int md_create(const int nd[3], const int *npts, const double *grids, const int bct[6],
+              const int periods[3], int so, double sigma, bool assemble, mf_md **out) {
   if (!nd || !npts || !grids || !bct || !out) return mf::fail(MF_ERR_ARG, "mf_md_create: null pointer");
   *out = nullptr;
   for (int a = 0; a < 3; ++a)
@@ -450,23 +471,26 @@ extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grid
     h->grids.assign(grids, grids + gofs);
   }
   for (int a = 0; a < 3; ++a) h->nd[a] = nd[a];
+  for (int a = 0; a < 3; ++a) h->periods[a] = (periods && periods[a]) ? 1 : 0;
   for (int f = 0; f < 6; ++f) h->bct[f] = bct[f];
   h->sigma = sigma;
   auto bail = [&](int rc) { delete h; return rc; };
   // interfaces (class_md.f90:1525-1559): i (x) outermost, k fastest; x, y, z
-  // interface of each subdomain in turn
+  // interface of each subdomain in turn, the high face of the last
+  // subdomain of a periodic axis included (it joins the first one's low face)
   struct Itf { int ax, i, j, k; long size; };
   std::vector<Itf> itfs;
   auto fsize = [&](int ax, int i, int j, int k) -> long {
     const long n1 = cnt[0][i] - 2, n2 = cnt[1][j] - 2, n3 = cnt[2][k] - 2;
     return ax == 0 ? n2 * n3 : (ax == 1 ? n1 * n3 : n1 * n2);
   };
+  const int *per = h->periods;
   for (int i = 0; i < nd[0]; ++i)
     for (int j = 0; j < nd[1]; ++j)
       for (int k = 0; k < nd[2]; ++k) {
-        if (i < nd[0] - 1) itfs.push_back({0, i, j, k, fsize(0, i, j, k)});
-        if (j < nd[1] - 1) itfs.push_back({1, i, j, k, fsize(1, i, j, k)});
-        if (k < nd[2] - 1) itfs.push_back({2, i, j, k, fsize(2, i, j, k)});
+        if (i < nd[0] - 1 || per[0]) itfs.push_back({0, i, j, k, fsize(0, i, j, k)});
+        if (j < nd[1] - 1 || per[1]) itfs.push_back({1, i, j, k, fsize(1, i, j, k)});
+        if (k < nd[2] - 1 || per[2]) itfs.push_back({2, i, j, k, fsize(2, i, j, k)});
       }
   h->ioff.assign(itfs.size() + 1, 0);
   for (size_t m = 0; m < itfs.size(); ++m) h->ioff[m + 1] = h->ioff[m] + itfs[m].size;
@@ -493,11 +517,11 @@ extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grid
         for (int f = 0; f < 6; ++f) {
           const int ax = f / 2, hi = f % 2;
           int m = -1;
-          if (!hi && id3[ax] > 0) {
+          if (!hi && (id3[ax] > 0 || per[ax])) {
             int lo[3] = {a, b, c};
-            lo[ax] -= 1;
+            lo[ax] = (lo[ax] + nd[ax] - 1) % nd[ax];
             m = find_itf(ax, lo[0], lo[1], lo[2]);
-          } else if (hi && id3[ax] < nd[ax] - 1) {
+          } else if (hi && (id3[ax] < nd[ax] - 1 || per[ax])) {
             m = find_itf(ax, a, b, c);
           }
           if (m < 0) continue;
@@ -563,7 +587,7 @@ extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grid
   if (e == hipSuccess && h->ninf) e = hipMalloc(&h->hist, 3 * (size_t)h->ninf * sizeof(double));
   if (e == hipSuccess && h->ninf) e = hipMemset(h->hist, 0, 3 * (size_t)h->ninf * sizeof(double));
   if (e != hipSuccess) return bail(mf::fail(MF_ERR_DEVICE, std::string("mf_md_create: ") + hipGetErrorString(e)));
-  if (mf::md::skip_assembly) {  // the caller fills the blocks (matrix cache)
+  if (!assemble) {  // the caller fills the blocks (matrix cache)
     *out = h;
     return MF_OK;
   }
@@ -615,6 +639,23 @@ extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grid
     if (rc != MF_OK) return bail(rc);
   }
   *out = h;
+  return MF_OK;
+}
+}  // namespace
+
+extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grids, const int bct[6],
+                            int so, double sigma, mf_md **out) {
+  return md_create(nd, npts, grids, bct, nullptr, so, sigma, true, out);
+}
+
+extern "C" int mf_md_set_null_space(mf_md *h, int psm) {
+  if (!h || psm < 0 || psm > 2) return mf::fail(MF_ERR_ARG, "mf_md_set_null_space: psm must be 0, 1 or 2");
+  if (psm == 1 && !h->ones && h->ninf) {
+    std::vector<double> one(h->ninf, 1.0);
+    MDCHK(hipMalloc(&h->ones, h->ninf * sizeof(double)));
+    MDCHK(hipMemcpy(h->ones, one.data(), h->ninf * sizeof(double), hipMemcpyHostToDevice));
+  }
+  h->psm = psm;
   return MF_OK;
 }
 
@@ -670,6 +711,16 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
   hipLaunchKernelGGL(mf::md::combine_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st,
                      (const double *)w.ys, (const long *)h->pa, (const long *)h->pb, n, w.r);
   MDCHK(hipGetLastError());
+  // singular systems: the constant part of the right-hand side removed
+  // (psm = 1, md_rhs_nullspace class_md.f90:347-357) or its first entry
+  // zeroed (psm = 2, md_vector_zero_lastpoint :251-267)
+  const bool nullsp = h->psm == 1;
+  if (nullsp) MDRC(remove_constant(h, w, w.r, st));
+  if (h->psm == 2) {
+    const double zero = 0.0;
+    MDCHK(hipMemcpyAsync(w.r, &zero, sizeof(double), hipMemcpyHostToDevice, st));
+    MDCHK(hipStreamSynchronize(st));
+  }
   // 2. GMRES(m) on M x = rhs, from x0 = 0 or, with the guess history on,
   //    x0 = s1 - 3 s2 + 3 s3 (class_md.f90:395-397)
   if (h->guess) {
@@ -697,6 +748,10 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
     MDRC(matvec(h, w, w.x, V, st));
     hipLaunchKernelGGL(mf::md::axpby_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st, 1.0,
                        (const double *)b, -1.0, V, n);
+    // with the constant null space every Krylov vector is kept orthogonal
+    // to the constant (KSPSetNullSpace, md_solve_init :318-328), so x keeps
+    // the mean of its start
+    if (nullsp) MDRC(remove_constant(h, w, V, st));
     double beta2 = 0.0;
     MDRC(dots(h, w, V, 1, V, &beta2, st));
     const double beta = std::sqrt(beta2);
@@ -710,6 +765,7 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
     for (; k < m && iters < max_iter; ++k) {
       double *vk1 = V + (long)(k + 1) * n;
       MDRC(matvec(h, w, V + (long)k * n, vk1, st));
+      if (nullsp) MDRC(remove_constant(h, w, vk1, st));
       // classical Gram-Schmidt, applied twice
       for (int pass = 0; pass < 2; ++pass) {
         double *hh = pass ? hc2.data() : hc.data();
@@ -762,6 +818,7 @@ extern "C" int mf_md_solve(mf_md *h, const double *const *g, double *const *u,
   MDRC(matvec(h, w, w.x, w.w, st));
   hipLaunchKernelGGL(mf::md::axpby_kernel, dim3(mf::md::nblk(n)), dim3(256), 0, st, 1.0,
                      (const double *)b, -1.0, w.w, n);
+  if (nullsp) MDRC(remove_constant(h, w, w.w, st));
   double rn2 = 0.0;
   MDRC(dots(h, w, w.w, 1, w.w, &rn2, st));
   const double res = bnorm > 0.0 ? std::sqrt(rn2) / bnorm : 0.0;
@@ -791,8 +848,8 @@ extern "C" int mf_md_set_guess(mf_md *h, int enable) {
 // format holding the layout's identity and the dense blocks.
 namespace {
 struct CacheHeader {
-  char magic[8];        // "MFMDBLK1"
-  int nd[3], bct[6], so, nslab;
+  char magic[8];        // "MFMDBLK2"
+  int nd[3], bct[6], periods[3], so, nslab;
   double sigma;
   unsigned long long grid_hash, ninf, doubles;
 };
@@ -806,8 +863,9 @@ unsigned long long fnv1a(const void *p, size_t n, unsigned long long h = 1469598
 CacheHeader header_of(const mf_md *h) {
   CacheHeader c;
   std::memset(&c, 0, sizeof(c));
-  std::memcpy(c.magic, "MFMDBLK1", 8);
+  std::memcpy(c.magic, "MFMDBLK2", 8);
   for (int a = 0; a < 3; ++a) c.nd[a] = h->nd[a];
+  for (int a = 0; a < 3; ++a) c.periods[a] = h->periods[a];
   for (int f = 0; f < 6; ++f) c.bct[f] = h->bct[f];
   c.so = h->so;
   c.nslab = (int)h->npts.size();
@@ -840,15 +898,14 @@ extern "C" int mf_md_write_matrix(const mf_md *h, const char *path) {
   return MF_OK;
 }
 
-extern "C" int mf_md_create_cached(const int nd[3], const int *npts, const double *grids,
-                                   const int bct[6], int so, double sigma, const char *path,
-                                   int *loaded, mf_md **out) {
-  if (!path || !out) return mf::fail(MF_ERR_ARG, "mf_md_create_cached: null pointer");
+extern "C" int mf_md_create_ex(const int nd[3], const int *npts, const double *grids,
+                               const int bct[6], const int periods[3], int so, double sigma,
+                               const char *path, int *loaded, mf_md **out) {
+  if (!out) return mf::fail(MF_ERR_ARG, "mf_md_create_ex: null pointer");
   if (loaded) *loaded = 0;
-  mf::md::skip_assembly = true;
+  if (!path) return md_create(nd, npts, grids, bct, periods, so, sigma, true, out);
   mf_md *h = nullptr;
-  int rc = mf_md_create(nd, npts, grids, bct, so, sigma, &h);
-  mf::md::skip_assembly = false;
+  int rc = md_create(nd, npts, grids, bct, periods, so, sigma, false, &h);
   if (rc != MF_OK) return rc;
   bool hit = false;
   if (FILE *f = std::fopen(path, "rb")) {
@@ -874,7 +931,7 @@ extern "C" int mf_md_create_cached(const int nd[3], const int *npts, const doubl
   // no usable cache: assemble and write it
   mf_md_destroy(h);
   h = nullptr;
-  rc = mf_md_create(nd, npts, grids, bct, so, sigma, &h);
+  rc = md_create(nd, npts, grids, bct, periods, so, sigma, true, &h);
   if (rc != MF_OK) return rc;
   rc = mf_md_write_matrix(h, path);
   if (rc != MF_OK) {
@@ -883,4 +940,11 @@ extern "C" int mf_md_create_cached(const int nd[3], const int *npts, const doubl
   }
   *out = h;
   return MF_OK;
+}
+
+extern "C" int mf_md_create_cached(const int nd[3], const int *npts, const double *grids,
+                                   const int bct[6], int so, double sigma, const char *path,
+                                   int *loaded, mf_md **out) {
+  if (!path || !out) return mf::fail(MF_ERR_ARG, "mf_md_create_cached: null pointer");
+  return mf_md_create_ex(nd, npts, grids, bct, nullptr, so, sigma, path, loaded, out);
 }

@@ -495,9 +495,13 @@ class Multidomain:
     fields; bc[s] is a device vector [bcx | bcy | bcz] (each Fortran-ordered)
     of subdomain s, outer faces used."""
 
-    def __init__(self, grids, sigma: float, bct=(1,) * 6, so: int = 6, cache=None):
+    def __init__(self, grids, sigma: float, bct=(1,) * 6, so: int = 6, cache=None,
+                 periods=(False, False, False), psm: int = 0):
         """cache: optional path of the influence-matrix file (read when it
-        matches this layout, else written after assembly; self.loaded tells)."""
+        matches this layout, else written after assembly; self.loaded tells).
+        periods: periodic axes (the last slab's high face joins the first
+        one's low face).  psm: the pressure singular method for singular
+        systems (0 none, 1 constant null space, 2 first rhs entry zeroed)."""
         lib = load()
         self._lib = lib
         self.nd = tuple(len(g) for g in grids)
@@ -511,16 +515,16 @@ class Multidomain:
                                  for a in range(self.nd[0])]]
         h = C.c_void_p()
         self.loaded = False
-        if cache is None:
-            check(lib.mf_md_create(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc), int(so),
-                                   float(sigma), C.byref(h)))
-        else:
-            ld = C.c_int()
-            check(lib.mf_md_create_cached(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc),
-                                          int(so), float(sigma), str(cache).encode(),
-                                          C.byref(ld), C.byref(h)))
-            self.loaded = bool(ld.value)
+        per = np.array([1 if p else 0 for p in periods], dtype=np.int32)
+        ld = C.c_int()
+        check(lib.mf_md_create_ex(_dptr(nd), _dptr(npts), _dptr(allg), _dptr(bc), _dptr(per),
+                                  int(so), float(sigma),
+                                  str(cache).encode() if cache is not None else None,
+                                  C.byref(ld), C.byref(h)))
+        self.loaded = bool(ld.value)
         self._h = h
+        if psm:
+            self.set_null_space(psm)
         nsub, ninf, nbytes = C.c_int(), C.c_longlong(), C.c_longlong()
         check(lib.mf_md_info(h, C.byref(nsub), C.byref(ninf), C.byref(nbytes)))
         self.nsub, self.ninf, self.matrix_bytes = nsub.value, ninf.value, nbytes.value
@@ -533,6 +537,10 @@ class Multidomain:
 
     def set_guess(self, enable: bool = True):
         check(self._lib.mf_md_set_guess(self._h, int(bool(enable))))
+
+    def set_null_space(self, psm: int):
+        """Pressure singular method psm (0, 1, 2) of the following solves."""
+        check(self._lib.mf_md_set_null_space(self._h, int(psm)))
 
     def write_matrix(self, path):
         check(self._lib.mf_md_write_matrix(self._h, str(path).encode()))

@@ -31,10 +31,38 @@ Here the interface system is solved densely.  The reference assumes one
 subdomain size; this restatement allows a point count per slab (the two
 subdomains of a face share their transverse slabs, so faces conform).
 
+Periodic axes (md_influence_matrix_param with periods, class_md.f90:1482-1560):
+along a periodic axis the high face of the last subdomain and the low face of
+the first are one more interface, numbered where the last subdomain's
+interface of that axis would stand in the loop; with one subdomain along
+that axis both of its faces carry the same interface.
+
+Singular systems (all outer faces Neumann or periodic, sigma = 0: adding a
+constant to every interface value adds it to every subdomain solution, so M
+has the constant vector in its null space), the pressure singular method
+psm (command_line.f90:27-29, navier_initialization :2119-2121):
+* psm = 0: the system as it is;
+* psm = 1: the constant is the system's null space: the constant part of
+  the right-hand side is removed (md_rhs_nullspace, class_md.f90:347-357)
+  and the Krylov solver works orthogonally to the constant (KSPSetNullSpace,
+  md_solve_init :318-328), so x has zero mean from a zero start and
+  M x = rhs - c 1 for a constant c;
+* psm = 2: the first entry of the right-hand side is set to zero
+  (md_vector_zero_lastpoint :251-267, applied to the pressure solve only,
+  multidomain_solve class_field.f90:1345-1352), and the system is solved as
+  it is.
+
 Pinned in tests/test_md.py: a polynomial that the compact schemes
 differentiate exactly is reproduced on every subdomain (so any error in the
 numbering, the sign rules or the assembly shows); the flux jumps vanish after
-the solve; one subdomain reduces to the plain solve.
+the solve; one subdomain reduces to the plain solve.  Periodic layouts: a
+periodic trigonometric solution converges at the schemes' order, the flux
+jumps vanish across the wrap interface too, and shifting the data by one
+slab along the periodic axis shifts the subdomain solutions (a wrong wrap
+numbering breaks all three).  psm = 1: M annihilates the constant, the flux
+jumps after the solve are one constant (zero for compatible data), x has
+zero mean; psm = 2: the first right-hand-side entry is zeroed (checked on a
+nonsingular layout against the dense solve of the modified system).
 """
 from __future__ import annotations
 
@@ -48,11 +76,12 @@ class Layout:
     (consecutive slabs share their end points); bct = outer boundary codes
     (x1,x2,y1,y2,z1,z2: 1 Dirichlet, 2 Neumann)."""
 
-    def __init__(self, grids, bct=(1,) * 6, so: int = 6):
+    def __init__(self, grids, bct=(1,) * 6, so: int = 6, periods=(False, False, False)):
         self.grids = [list(g) for g in grids]
         self.nd = tuple(len(g) for g in grids)
         self.bct = tuple(bct)
         self.so = so
+        self.periods = tuple(bool(p) for p in periods)
         for ax in range(3):
             for k in range(self.nd[ax] - 1):
                 if self.grids[ax][k][-1] != self.grids[ax][k + 1][0]:
@@ -65,12 +94,9 @@ class Layout:
         for i in range(self.nd[0]):
             for j in range(self.nd[1]):
                 for k in range(self.nd[2]):
-                    if i < self.nd[0] - 1:
-                        self.interfaces.append((0, (i, j, k)))
-                    if j < self.nd[1] - 1:
-                        self.interfaces.append((1, (i, j, k)))
-                    if k < self.nd[2] - 1:
-                        self.interfaces.append((2, (i, j, k)))
+                    for ax, q in enumerate((i, j, k)):
+                        if q < self.nd[ax] - 1 or self.periods[ax]:
+                            self.interfaces.append((ax, (i, j, k)))
         self.iid = {itf: m for m, itf in enumerate(self.interfaces)}
         self.isize = [self.face_size(ax, low) for ax, low in self.interfaces]
         self.ioff = np.concatenate([[0], np.cumsum(self.isize)]).astype(int)
@@ -86,22 +112,21 @@ class Layout:
     def sub_bct(self, s):
         """Boundary codes of subdomain s: interfaces are Dirichlet."""
         b = list(self.bct)
-        for ax in range(3):
-            if s[ax] > 0:
-                b[2 * ax] = 1
-            if s[ax] < self.nd[ax] - 1:
-                b[2 * ax + 1] = 1
+        for f, _ in self.faces(s):
+            b[f] = 1
         return tuple(b)
 
     def faces(self, s):
-        """[(face, interface id)] of subdomain s, faces ascending."""
+        """[(face, interface id)] of subdomain s, faces ascending; on a
+        periodic axis the first subdomain's low face is the last one's high
+        interface."""
         out = []
         for ax in range(3):
-            if s[ax] > 0:
+            if s[ax] > 0 or self.periods[ax]:
                 low = list(s)
-                low[ax] -= 1
+                low[ax] = (s[ax] - 1) % self.nd[ax]
                 out.append((2 * ax, self.iid[(ax, tuple(low))]))
-            if s[ax] < self.nd[ax] - 1:
+            if s[ax] < self.nd[ax] - 1 or self.periods[ax]:
                 out.append((2 * ax + 1, self.iid[(ax, s)]))
         out.sort()
         return out
@@ -128,11 +153,15 @@ def _face_normal_derivatives(dcs, u):
 
 
 class Multidomain:
-    def __init__(self, layout: Layout, sigma: float, assemble: bool = True):
+    def __init__(self, layout: Layout, sigma: float, assemble: bool = True, psm: int = 0):
         """assemble=False skips the dense influence matrix (rhs and
         flux_jumps do not need it; solve does), for layouts too large for a
-        dense ninf x ninf matrix."""
+        dense ninf x ninf matrix.  psm: the pressure singular method (0, 1,
+        2; see the module docstring)."""
+        if psm not in (0, 1, 2):
+            raise ValueError("psm must be 0, 1 or 2")
         self.L = layout
+        self.psm = psm
         self.sigma = float(sigma)
         self.sc, self.dcs = {}, {}
         for s in layout.subs:
@@ -182,10 +211,28 @@ class Multidomain:
                 r[L.ioff[J]:L.ioff[J + 1]] += d[sl[f]]
         return r
 
+    def interface_values(self, r):
+        """x from the right-hand side r (KSPSolve with the psm treatment)."""
+        n = self.L.ninf
+        if n == 0:
+            return np.zeros(0)
+        if self.psm == 1:
+            # (I - P) M x = (I - P) r with x orthogonal to the constant
+            # (P: projection on the constant), solved as one consistent
+            # least-squares system
+            rp = r - r.mean()
+            Mp = self.M - self.M.mean(axis=0, keepdims=True)
+            A = np.vstack([Mp, np.ones((1, n))])
+            return np.linalg.lstsq(A, np.concatenate([rp, [0.0]]), rcond=None)[0]
+        r = r.copy()
+        if self.psm == 2:
+            r[0] = 0.0
+        return np.linalg.solve(self.M, r)
+
     def solve(self, g, bc):
         """Interface values x and the subdomain solutions {s: u}."""
         r = self.rhs(g, bc)
-        x = np.linalg.solve(self.M, r) if self.L.ninf else np.zeros(0)
+        x = self.interface_values(r)
         u = {}
         for s in self.L.subs:
             us = np.zeros(self.L.shape(s), order="F")

@@ -7,7 +7,8 @@ The oracle (oracle/md_oracle.py) is pinned by what the mathematics fixes:
   md_influence_matrix_set / multidomain_solve or the assembly would show;
 * after the solve the flux jumps (sum over both sides of the outward-normal
   first derivatives) vanish, for a non-polynomial solution too;
-* with one subdomain there are no interfaces and the solve is the plain one.
+* with one subdomain there are no interfaces and the solve is the plain one;
+* periodic layouts and the null-space options: see the oracle tests below.
 The GPU solve (mf_md_*) is compared with the oracle on the same layouts: the
 influence matrix through M x, the interface values and every subdomain
 field, to 1e-9 relative (GMRES stops at 1e-12).
@@ -103,6 +104,137 @@ def test_oracle_single_domain_is_plain_solve():
     assert np.abs(u[s] - ref).max() == 0.0
 
 
+def layout_periodic(n=12):
+    # periodic along x (2 slabs, the wrap interface joins x = 1 to x = 0) and
+    # z (one slab whose two faces are one interface), walls along y
+    return [[o.grid_uniform(n, 0, 0.5), o.grid_uniform(n, 0.5, 1.0)],
+            [o.grid_asin(n, -1.0, 1.0)],
+            [o.grid_uniform(n + 1, 0, 1.0)]]
+
+
+def trig(X, Y, Z):
+    return np.sin(2 * np.pi * X + 0.4) * np.cos(0.5 * np.pi * Y) * np.cos(2 * np.pi * Z) + 0.3 * np.cos(2 * np.pi * X)
+
+
+def trig_lap(X, Y, Z):
+    a = np.sin(2 * np.pi * X + 0.4) * np.cos(0.5 * np.pi * Y) * np.cos(2 * np.pi * Z)
+    return -(8.25 * np.pi ** 2) * a - 0.3 * 4 * np.pi ** 2 * np.cos(2 * np.pi * X)
+
+
+def neumann_trig(X, Y, Z):
+    # d/dy = 0 at y = -1, 1: a compatible all-Neumann/periodic pressure
+    return np.cos(2 * np.pi * X) * np.cos(np.pi * Y) * np.cos(2 * np.pi * Z + 0.2) + np.sin(2 * np.pi * X) * np.cos(np.pi * Y)
+
+
+def neumann_trig_lap(X, Y, Z):
+    a = np.cos(2 * np.pi * X) * np.cos(np.pi * Y) * np.cos(2 * np.pi * Z + 0.2)
+    b = np.sin(2 * np.pi * X) * np.cos(np.pi * Y)
+    return -9.0 * np.pi ** 2 * a - 5.0 * np.pi ** 2 * b
+
+
+def _neumann_data(L, fun):
+    """Outer-face arrays with the exact normal derivative on the y walls (the
+    solver's Neumann data, class_solver_3d.f90: du/dn values), interface
+    faces left for the solve."""
+    g, bc = md.manufactured(L, fun, lambda X, Y, Z: 0.0 * X, 0.0)
+    h = 1e-6
+    for s in L.subs:
+        gx, gy, gz = (np.asarray(L.grids[ax][s[ax]]) for ax in range(3))
+        X, Y, Z = np.meshgrid(gx[1:-1], gy[[0, -1]], gz[1:-1], indexing="ij")
+        dy = (fun(X, Y + h, Z) - fun(X, Y - h, Z)) / (2 * h)
+        bcx, bcy, bcz = bc[s]
+        bcy = np.asfortranarray(dy.transpose(0, 2, 1))
+        bc[s] = (bcx, bcy, bcz)
+    return bc
+
+
+@pytest.mark.parametrize("sigma", [-5.0, 3.0])
+def test_oracle_periodic_converges(sigma):
+    """Periodic along x and z: the solution of a periodic trigonometric
+    problem converges with the grid (error below 5e-3 at 12 points per slab
+    and falling by > 6 from 12 to 20, as the compact boundary closures
+    allow; the same solution with Dirichlet data on every face falls by 9.5),
+    and the flux jumps vanish at every interface, the wrap interfaces
+    included."""
+    errs = []
+    for n in (12, 20):
+        L = md.Layout(layout_periodic(n), bct=(1, 1, 1, 1, 1, 1), periods=(True, False, True))
+        assert L.ninf == 2 * (n - 2) * (n - 1) + 2 * (n - 2) * (n - 2)
+        mdo = md.Multidomain(L, sigma)
+        g, bc = md.manufactured(L, trig, trig_lap, sigma)
+        x, u = mdo.solve(g, bc)
+        j = mdo.flux_jumps(u)
+        assert np.abs(j).max() < 1e-10 * np.abs(mdo.rhs(g, bc)).max()
+        errs.append(_exact_err(L, u, trig))
+    print("periodic errors", errs)
+    assert errs[0] < 5e-3 and errs[0] / errs[1] > 6.0, errs
+
+
+def test_oracle_periodic_shift():
+    """Data shifted by one slab along the periodic x axis give subdomain
+    solutions shifted by one slab (x values differ by 0.5 only through the
+    slabs' grids, so to rounding)."""
+    grids = layout_periodic(12)
+    L = md.Layout(grids, periods=(True, False, True))
+    mdo = md.Multidomain(L, -5.0)
+    g, bc = md.manufactured(L, trig, trig_lap, -5.0)
+    shifted = lambda X, Y, Z: trig(X + 0.5, Y, Z)
+    shifted_lap = lambda X, Y, Z: trig_lap(X + 0.5, Y, Z)
+    g2, bc2 = md.manufactured(L, shifted, shifted_lap, -5.0)
+    _, u = mdo.solve(g, bc)
+    _, u2 = mdo.solve(g2, bc2)
+    scale = max(np.abs(v).max() for v in u.values())
+    for s in L.subs:
+        t = ((s[0] + 1) % 2, s[1], s[2])
+        assert np.abs(u2[s] - u[t]).max() < 1e-11 * scale
+
+
+def test_oracle_null_space_psm1():
+    """All outer faces Neumann or periodic with sigma = 0: M annihilates the
+    constant; with psm = 1 the interface values have zero mean, the flux
+    jumps after the solve are one constant (the right-hand side's constant
+    part, ~0 for compatible data) and the solution is the exact one up to a
+    constant and the discretisation error."""
+    grids = layout_periodic(14)
+    L = md.Layout(grids, bct=(2, 2, 2, 2, 2, 2), periods=(True, False, True))
+    mdo = md.Multidomain(L, 0.0, psm=1)
+    ones = np.ones(L.ninf)
+    assert np.abs(mdo.M @ ones).max() < 1e-10 * np.abs(mdo.M).max()
+    g, _ = md.manufactured(L, neumann_trig, neumann_trig_lap, 0.0)
+    bc = _neumann_data(L, neumann_trig)
+    r = mdo.rhs(g, bc)
+    x, u = mdo.solve(g, bc)
+    assert abs(x.mean()) < 1e-13 * np.abs(x).max()
+    j = mdo.flux_jumps(u)
+    assert np.abs(j - j.mean()).max() < 1e-10 * np.abs(r).max()
+    assert abs(j.mean()) < 1e-3 * np.abs(r).max()
+    # compare with the exact solution up to a constant
+    W = []
+    for s in L.subs:
+        gx, gy, gz = (np.asarray(L.grids[ax][s[ax]]) for ax in range(3))
+        X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+        W.append((u[s] - neumann_trig(X, Y, Z))[1:-1, 1:-1, 1:-1])
+    off = np.mean(np.concatenate([w.ravel() for w in W]))
+    err = max(np.abs(w - off).max() for w in W)
+    print("psm=1: flux-jump constant %.2e, error up to a constant %.2e" % (j.mean(), err))
+    assert err < 5e-3  # discretisation level, as the periodic case above
+
+
+def test_oracle_psm2_zeroes_first_rhs_entry():
+    """psm = 2 sets the first right-hand-side entry to zero: on a nonsingular
+    layout, the flux jumps of the resulting solution are zero except at
+    the first interface unknown, where the jump is the removed entry."""
+    L = md.Layout(layout_small())
+    mdo = md.Multidomain(L, 2.0, psm=2)
+    g, bc = md.manufactured(L, sines, sines_lap, 2.0)
+    r = mdo.rhs(g, bc)
+    _, u = mdo.solve(g, bc)
+    j = mdo.flux_jumps(u)
+    # the jumps are r - M x (zero when M x = r), so here (r[0], 0, ..., 0)
+    assert abs(j[0] - r[0]) < 1e-10 * np.abs(r).max() and abs(r[0]) > 1e-3 * np.abs(r).max()
+    assert np.abs(j[1:]).max() < 1e-10 * np.abs(r).max()
+
+
 # ------------------------------------------------------------------- GPU ----
 
 def _mf():
@@ -152,6 +284,92 @@ def test_gpu_md_matches_oracle(lay, sigma, fun, lap):
     assert e_u / scale < 1e-9
     if fun is poly:
         assert _exact_err(L, {s: mf.to_host(u[i]) for i, s in enumerate(L.subs)}, poly) < 1e-9
+
+
+def _gpu_vs_oracle(mf, torch, grids, sigma, g, bc, L, mdo, tol=1e-12, **kw):
+    """Device solve of the layout against the oracle's: (e_M, e_x, e_u, report, u)."""
+    xo, uo = mdo.solve(g, bc)
+    h = mf.Multidomain(grids, sigma, **kw)
+    assert h.ninf == L.ninf and h.nsub == len(L.subs)
+    rng = np.random.default_rng(7)
+    xr = rng.standard_normal(L.ninf)
+    y = mf.to_host(h.matvec(torch.tensor(xr, device="cuda")))
+    ref = mdo.M @ xr
+    e_m = np.abs(y - ref).max() / np.abs(ref).max()
+    gd = [mf.to_device(g[s]) for s in L.subs]
+    bd = [torch.tensor(_bc_vec(bc[s]), device="cuda") for s in L.subs]
+    xi = torch.zeros(L.ninf, dtype=torch.float64, device="cuda")
+    u, rep = h.solve(gd, bd, tol=tol, x_iface=xi)
+    e_x = np.abs(mf.to_host(xi) - xo).max() / np.abs(xo).max()
+    uh = {s: mf.to_host(u[i]) for i, s in enumerate(L.subs)}
+    scale = max(np.abs(uo[s]).max() for s in L.subs)
+    e_u = max(np.abs(uh[s] - uo[s])[1:-1, 1:-1, 1:-1].max() for s in L.subs) / scale
+    print("M x %.2e  x %.2e  u %.2e  GMRES %d its, residual %.2e"
+          % (e_m, e_x, e_u, rep.iterations, rep.residual), flush=True)
+    return e_m, e_x, e_u, rep, uh, h
+
+
+@pytest.mark.gpu
+def test_gpu_md_periodic_matches_oracle(tmp_path):
+    """Periodic along x (two slabs, wrap interface) and z (one slab whose two
+    faces share an interface), Dirichlet walls along y: the device matrix,
+    interface values and fields against the oracle, as for the non-periodic
+    layouts; the matrix cache records the periods."""
+    mf, torch = _mf()
+    grids = layout_periodic(12)
+    per = (True, False, True)
+    sigma = -5.0
+    L = md.Layout(grids, periods=per)
+    mdo = md.Multidomain(L, sigma)
+    g, bc = md.manufactured(L, trig, trig_lap, sigma)
+    print("run md periodic layout ninf %d" % L.ninf, flush=True)
+    path = tmp_path / "periodic.bin"
+    e_m, e_x, e_u, rep, uh, h = _gpu_vs_oracle(mf, torch, grids, sigma, g, bc, L, mdo,
+                                               periods=per, cache=path)
+    assert rep.residual <= 1e-11 and e_m < 1e-11 and e_x < 1e-9 and e_u < 1e-9
+    j = mdo.flux_jumps(uh)
+    assert np.abs(j).max() < 1e-9 * np.abs(mdo.rhs(g, bc)).max()
+    h2 = mf.Multidomain(grids, sigma, periods=per, cache=path)
+    assert h2.loaded
+    h3 = mf.Multidomain(grids, sigma, periods=(True, False, False), cache=path)
+    assert not h3.loaded and h3.ninf != h2.ninf
+
+
+@pytest.mark.gpu
+def test_gpu_md_null_space_psm1():
+    """sigma = 0 with Neumann walls and periodic x, z: the interface system
+    is singular; psm = 1 solves it orthogonally to the constant on the
+    device as in the oracle (zero-mean interface values), the flux jumps of
+    the device solution are one constant."""
+    mf, torch = _mf()
+    grids = layout_periodic(14)
+    per = (True, False, True)
+    L = md.Layout(grids, bct=(2, 2, 2, 2, 2, 2), periods=per)
+    mdo = md.Multidomain(L, 0.0, psm=1)
+    g, _ = md.manufactured(L, neumann_trig, neumann_trig_lap, 0.0)
+    bc = _neumann_data(L, neumann_trig)
+    print("run md psm=1 layout ninf %d" % L.ninf, flush=True)
+    e_m, e_x, e_u, rep, uh, h = _gpu_vs_oracle(mf, torch, grids, 0.0, g, bc, L, mdo,
+                                               bct=(2, 2, 2, 2, 2, 2), periods=per, psm=1)
+    assert rep.residual <= 1e-11 and e_m < 1e-11 and e_x < 1e-9 and e_u < 1e-9
+    j = mdo.flux_jumps(uh)
+    r = mdo.rhs(g, bc)
+    assert np.abs(j - j.mean()).max() < 1e-9 * np.abs(r).max()
+
+
+@pytest.mark.gpu
+def test_gpu_md_psm2():
+    """psm = 2 zeroes the first right-hand-side entry: on a nonsingular
+    layout the device solution equals the oracle's solution of that
+    modified system."""
+    mf, torch = _mf()
+    grids = layout_small()
+    L = md.Layout(grids)
+    mdo = md.Multidomain(L, 2.0, psm=2)
+    g, bc = md.manufactured(L, sines, sines_lap, 2.0)
+    print("run md psm=2 layout ninf %d" % L.ninf, flush=True)
+    e_m, e_x, e_u, rep, uh, h = _gpu_vs_oracle(mf, torch, grids, 2.0, g, bc, L, mdo, psm=2)
+    assert rep.residual <= 1e-11 and e_x < 1e-9 and e_u < 1e-9
 
 
 @pytest.mark.gpu
