@@ -235,6 +235,13 @@ int mf_md_create_ex(const int nd[3], const int *npts, const double *grids, const
  *    to zero (md_vector_zero_lastpoint :251-267; the reference does so on the
  *    pressure solve, the one caller that passes null).  Default 0. */
 int mf_md_set_null_space(mf_md *h, int psm);
+/* No counterpart in the reference: the measurement of the autotuner
+ * (mflops3d-md_amd/tune/infl_autotune.py) that chooses the tile shape of the
+ * grouped influence-block GEMM: one M x N x K block product through that
+ * kernel with tile class tile (0: 256x64, 1: 128x128, 2: 64x256) and
+ * blocks_per_cu persistent blocks per compute unit; mean ms per launch over
+ * reps launches.  Allocates its operands. */
+int mf_tune_infl_gemm(int tile, int blocks_per_cu, int M, int N, int K, int reps, double *ms);
 
 /* Kernel timing (no counterpart in the reference; its solve_3d has
  * commented-out cpu_time probes, class_solver_3d.f90:106-259).  While

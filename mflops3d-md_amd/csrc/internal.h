@@ -6,6 +6,7 @@
 #include <vector>
 
 struct mf_der;
+struct mf_solver3d;
 
 namespace mf {
 
@@ -41,6 +42,23 @@ bool solver_setup_1d(const double *grid, int nall, const int bct[2], int so, Sol
 // scheme (dl_lu after pentalu, dr), i.e. the boundary-point derivative as a
 // dot product with the whole line.
 void derivative_rows(int n, const double *dl_lu, const double *dr, double *row0, double *rown);
+
+// Influence blocks of a set of subdomains by spectral contraction
+// (solve.hip): for every pair of interface faces (f_in, f_out) of a
+// subdomain, the block of its dense u x u matrix D (column-major) at rows
+// loc_off[f_out].., columns loc_off[f_in]..  Phase-A kernels per subdomain,
+// then one persistent grouped GEMM launch per tile class, captured in a
+// HIP graph and launched once when graph is true.
+struct InflSub {
+  struct ::mf_solver3d *sc;
+  const struct ::mf_der *d[3];
+  int nf;
+  int face[6];
+  long loc_off[7];
+  long u;
+  double *D;
+};
+int influence_assemble(const std::vector<InflSub> &subs, double sigma, bool graph, void *stream);
 
 // host copies of a derivative handle's tables (order 1|2, sub 0|1)
 bool der_host_tables(const struct ::mf_der *dc, int order, int sub, const double **dl_lu,

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -591,7 +592,31 @@ int md_create(const int nd[3], const int *npts, const double *grids, const int b
     *out = h;
     return MF_OK;
   }
-  // influence blocks from the batched unit-impulse responses
+  // influence blocks by spectral contraction (one persistent grouped launch,
+  // captured in a HIP graph), or, with MF_INFL_IMPULSES=1, from batched
+  // unit-impulse solves as the reference forms them
+  const char *imp_env = std::getenv("MF_INFL_IMPULSES");
+  if (!(imp_env && imp_env[0] == '1')) {
+    std::vector<mf::InflSub> is;
+    for (auto &s : h->subs) {
+      if (!s.u) continue;
+      mf::InflSub q;
+      q.sc = s.sc;
+      for (int a = 0; a < 3; ++a) q.d[a] = s.d[a];
+      q.nf = s.nf;
+      for (int f = 0; f < s.nf; ++f) q.face[f] = s.face[f];
+      for (int f = 0; f <= s.nf; ++f) q.loc_off[f] = s.loc_off[f];
+      q.u = s.u;
+      q.D = h->D + s.D_off;
+      is.push_back(q);
+    }
+    const char *g_env = std::getenv("MF_INFL_GRAPH");
+    const bool graph = !(g_env && g_env[0] == '0');
+    const int rc = mf::influence_assemble(is, sigma, graph, nullptr);
+    if (rc != MF_OK) return bail(rc);
+    *out = h;
+    return MF_OK;
+  }
   {
     const long chunk = 1024;
     long maxstride = 0;

@@ -30,6 +30,8 @@
 #include "gemm_h2.h"
 #include "gemm_h2r.h"
 #include "gemm_h2z.h"
+#include "infl_spec.h"
+#include "infl_tune.h"
 #include "influence.h"
 #include "internal.h"
 #include "residual.h"
@@ -1354,5 +1356,464 @@ extern "C" int mf_influence_impulses(mf_solver3d *h, const mf_der *dx, const mf_
     }
     HIPCHK(hipGetLastError());
   }
+  return MF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Influence blocks by spectral contraction (infl_spec.h): the influence
+// matrix of md_influence_matrix_set (class_field.f90:1396-1510) without one
+// solve per impulse
+// ---------------------------------------------------------------------------
+namespace {
+
+// ahat = inv_vect_s (-fold) of impulse face (s, side): the spectral
+// coefficients of a unit boundary value folded into the right-hand side
+std::vector<double> impulse_ahat(const mf_solver3d *h, int s, int side) {
+  const mf::Solver1D &S1 = h->ax[s];
+  const int ns = S1.n;
+  std::vector<double> fold(ns), ahat(ns, 0.0);
+  auto IL = [&](int i, int j) { return S1.inv_lhs[i + (size_t)ns * j]; };
+  auto DR = [&](int i, int c) { return S1.ddr[i + (size_t)ns * c]; };
+  for (int i = 0; i < ns; ++i)
+    fold[i] = side == 0 ? IL(i, 0) * DR(0, 0) + IL(i, 1) * DR(1, 0)
+                        : IL(i, ns - 2) * DR(ns - 2, 7) + IL(i, ns - 1) * DR(ns - 1, 6);
+  for (int p = 0; p < ns; ++p) {
+    double acc = 0.0;
+    for (int i = 0; i < ns; ++i) acc += S1.inv_vect[p + (size_t)ns * i] * (-fold[i]);
+    ahat[p] = acc;
+  }
+  return ahat;
+}
+
+// Host tables of one axis: the boundary-derivative rows of the two faces
+// with the Neumann extrapolation of the line folded in (as the face
+// derivatives of the impulse path form them), in spectral space (what), the
+// products Om = diag(what) W, and the Dirichlet end weights.
+struct AxisTabs {
+  int n = 0;
+  std::vector<double> what[2], Om[2];
+  double dcoef[2][2];  // [output side][impulse side]
+};
+
+bool axis_tabs(const mf_solver3d *h, const mf_der *d, int a, AxisTabs &t) {
+  const mf::Solver1D &S = h->ax[a];
+  const int n = S.n;
+  t.n = n;
+  const double *dl = nullptr, *dr = nullptr;
+  int nall = 0;
+  if (!mf::der_host_tables(d, 1, 0, &dl, &dr, &nall) || nall != n + 2) return false;
+  std::vector<double> r0(nall), rn(nall);
+  mf::derivative_rows(nall, dl, dr, r0.data(), rn.data());
+  // boundary value of a Neumann end as a combination of the first/last six
+  // interior values (extrapolation_coeffs, class_solver_3d.f90:622-656)
+  std::vector<double> eL(n, 0.0), eH(n, 0.0);
+  const double *nm = S.neum.data();
+  if (S.bc[0] != 1)
+    for (int m = 1; m < 7; ++m) eL[m - 1] = -nm[2 * m] / nm[0];
+  if (S.bc[1] != 1)
+    for (int m = 0; m < 6; ++m) eH[n - 6 + m] = -nm[1 + 2 * m] / nm[13];
+  std::vector<double> w[2] = {std::vector<double>(n), std::vector<double>(n)};
+  for (int i = 0; i < n; ++i) {
+    w[0][i] = r0[i + 1] + r0[0] * eL[i] + r0[n + 1] * eH[i];
+    w[1][i] = -(rn[i + 1] + rn[0] * eL[i] + rn[n + 1] * eH[i]);
+  }
+  for (int side = 0; side < 2; ++side) {
+    t.what[side].assign(n, 0.0);
+    for (int al = 0; al < n; ++al) {
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) acc += w[side][i] * S.vect[i + (size_t)n * al];
+      t.what[side][al] = acc;
+    }
+    t.Om[side].resize((size_t)n * n);
+    for (int c = 0; c < n; ++c)
+      for (int al = 0; al < n; ++al)
+        t.Om[side][al + (size_t)n * c] = t.what[side][al] * S.inv_vect[al + (size_t)n * c];
+  }
+  t.dcoef[0][0] = r0[0];
+  t.dcoef[0][1] = r0[n + 1];
+  t.dcoef[1][0] = -rn[0];
+  t.dcoef[1][1] = -rn[n + 1];
+  return true;
+}
+
+// device offsets (doubles) of one subdomain's uploaded tables
+struct SubTabs {
+  size_t V[3], W[3], what[3][2], Om[3][2];
+  size_t ahat[6], VsA[6], VsT[6];
+  double dcoef[3][2][2];
+};
+
+// one face pair of one subdomain
+struct PairPlan {
+  int sub, fi, fo;          // subdomain; impulse and output face (0..5)
+  int ii, io;               // their positions among the subdomain's interface faces
+  size_t buf1, buf2;        // work offsets: G or H, then Z or Y
+  int tile;
+};
+
+int tile_bm(int tile) { return tile == 0 ? 256 : (tile == 1 ? 128 : 64); }
+
+template <int BM, int BN>
+void launch_infl(const mf::InflProblem *probs, int nprob, long tiles, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL((mf::infl_gemm_kernel<BM, BN>), dim3(blocks), dim3(256), 0, st, probs, nprob,
+                     tiles);
+}
+
+void launch_infl_tile(int tile, const mf::InflProblem *probs, int nprob, long tiles, int blocks,
+                      hipStream_t st) {
+  if (tile == 0) launch_infl<256, 64>(probs, nprob, tiles, blocks, st);
+  else if (tile == 1) launch_infl<128, 128>(probs, nprob, tiles, blocks, st);
+  else launch_infl<64, 256>(probs, nprob, tiles, blocks, st);
+}
+
+}  // namespace
+
+int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool graph, void *stream) {
+  using mf::InflProblem;
+  if (subs.empty()) return MF_OK;
+  const int nsub = (int)subs.size();
+  // ---- host tables and their device layout
+  std::vector<double> htab;
+  std::vector<SubTabs> tabs(nsub);
+  auto put = [&](const std::vector<double> &v) {
+    const size_t o = htab.size();
+    htab.insert(htab.end(), v.begin(), v.end());
+    return o;
+  };
+  for (int i = 0; i < nsub; ++i) {
+    const InflSub &S = subs[i];
+    const mf_solver3d *h = S.sc;
+    for (int a = 0; a < 3; ++a) {
+      AxisTabs at;
+      if (!axis_tabs(h, S.d[a], a, at)) return mf::fail(MF_ERR_SETUP, "influence_assemble: derivative tables");
+      tabs[i].V[a] = put(h->ax[a].vect);
+      tabs[i].W[a] = put(h->ax[a].inv_vect);
+      for (int side = 0; side < 2; ++side) {
+        tabs[i].what[a][side] = put(at.what[side]);
+        tabs[i].Om[a][side] = put(at.Om[side]);
+        for (int is = 0; is < 2; ++is) tabs[i].dcoef[a][side][is] = at.dcoef[side][is];
+      }
+    }
+    for (int f = 0; f < S.nf; ++f) {
+      const int face = S.face[f], s = face / 2, n = h->ax[s].n;
+      const std::vector<double> ah = impulse_ahat(h, s, face % 2);
+      std::vector<double> va((size_t)n * n), vt((size_t)n * n);
+      for (int al = 0; al < n; ++al)
+        for (int o = 0; o < n; ++o) {
+          const double v = h->ax[s].vect[o + (size_t)n * al] * ah[al];
+          va[o + (size_t)n * al] = v;
+          vt[al + (size_t)n * o] = v;
+        }
+      tabs[i].ahat[face] = put(ah);
+      tabs[i].VsA[face] = put(va);
+      tabs[i].VsT[face] = put(vt);
+    }
+  }
+  // ---- work layout per subdomain: T, TT per axis, then two buffers per pair
+  std::vector<PairPlan> pairs;
+  std::vector<size_t> sub_work(nsub), sub_T(nsub);
+  for (int i = 0; i < nsub; ++i) {
+    const InflSub &S = subs[i];
+    const int n[3] = {S.sc->n1, S.sc->n2, S.sc->n3};
+    size_t off = 0;
+    sub_T[i] = 0;
+    for (int a = 0; a < 3; ++a) off += 2 * (size_t)n[a] * n[a] * n[a];
+    for (int fi = 0; fi < S.nf; ++fi)
+      for (int fo = 0; fo < S.nf; ++fo) {
+        PairPlan pp;
+        pp.sub = i; pp.ii = fi; pp.io = fo; pp.fi = S.face[fi]; pp.fo = S.face[fo];
+        const int s = pp.fi / 2, t = pp.fo / 2;
+        size_t b1, b2;
+        if (s == t) {
+          const int b = s == 0 ? 1 : 0, c = s == 2 ? 1 : 2;
+          b1 = 2 * (size_t)n[b] * n[c];
+          b2 = (size_t)n[b] * n[c] * n[c];
+        } else {
+          b1 = b2 = (size_t)n[0] * n[1] * n[2];
+        }
+        pp.buf1 = off; off += b1;
+        pp.buf2 = off; off += b2;
+        pairs.push_back(pp);
+      }
+    sub_work[i] = off;
+  }
+  // ---- groups of consecutive subdomains within a work budget
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  size_t budget = free_b / 4 / sizeof(double);
+  size_t need_max = 0;
+  for (int i = 0; i < nsub; ++i) need_max = std::max(need_max, sub_work[i]);
+  budget = std::max(budget, need_max);
+  std::vector<int> group_of(nsub);
+  std::vector<size_t> sub_base(nsub);
+  size_t work_doubles = 0;
+  {
+    int g = 0;
+    size_t used = 0;
+    for (int i = 0; i < nsub; ++i) {
+      if (used + sub_work[i] > budget && used > 0) { ++g; used = 0; }
+      group_of[i] = g;
+      sub_base[i] = used;
+      used += sub_work[i];
+      work_doubles = std::max(work_doubles, used);
+    }
+  }
+  const int ngroups = group_of[nsub - 1] + 1;
+  // ---- problem descriptors: per group, per tile class, contiguous
+  int ncu = 256;
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      int v = 0;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
+    }
+  }
+  // pointers into buffers that do not exist yet: record offsets, patch below
+  struct ProbRef { int pair; int cls; };
+  std::vector<std::vector<ProbRef>> order(ngroups * 3);
+  for (size_t q = 0; q < pairs.size(); ++q) {
+    PairPlan &pp = pairs[q];
+    const InflSub &S = subs[pp.sub];
+    const int n[3] = {S.sc->n1, S.sc->n2, S.sc->n3};
+    const int s = pp.fi / 2, t = pp.fo / 2;
+    int K;
+    if (s == t) K = n[s == 0 ? 1 : 0];
+    else K = n[3 - s - t];
+    pp.tile = mf::infl_tile_for(K).tile;
+    order[group_of[pp.sub] * 3 + pp.tile].push_back({(int)q, pp.tile});
+  }
+  // ---- device buffers
+  double *dtab = nullptr, *dwork = nullptr;
+  InflProblem *dprob = nullptr;
+  struct Launch { int tile; long first; int nprob; long tiles; int blocks; };
+  std::vector<std::vector<Launch>> launches(ngroups);
+  std::vector<InflProblem> hprob;
+  int rc = MF_OK;
+  hipStream_t cs = nullptr;
+  hipGraph_t gr = nullptr;
+  hipGraphExec_t ge = nullptr;
+  auto cleanup = [&]() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (cs) { (void)hipStreamSynchronize(cs); (void)hipStreamDestroy(cs); }
+    if (dtab) (void)hipFree(dtab);
+    if (dwork) (void)hipFree(dwork);
+    if (dprob) (void)hipFree(dprob);
+  };
+  auto fail_dev = [&](hipError_t e, const char *what) {
+    cleanup();
+    return mf::fail(MF_ERR_DEVICE, std::string("influence_assemble: ") + what + ": " + hipGetErrorString(e));
+  };
+  hipError_t e = hipMalloc(&dtab, htab.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dwork, std::max<size_t>(work_doubles, 1) * sizeof(double));
+  if (e != hipSuccess) return fail_dev(e, "hipMalloc");
+  for (int g = 0; g < ngroups; ++g)
+    for (int cls = 0; cls < 3; ++cls) {
+      const std::vector<ProbRef> &lst = order[g * 3 + cls];
+      if (lst.empty()) continue;
+      Launch L;
+      L.tile = cls;
+      L.first = (long)hprob.size();
+      L.nprob = (int)lst.size();
+      long tiles = 0;
+      const int BM = tile_bm(cls), BN = 256 * 64 / BM;
+      for (const ProbRef &r : lst) {
+        const PairPlan &pp = pairs[r.pair];
+        const InflSub &S = subs[pp.sub];
+        const int n[3] = {S.sc->n1, S.sc->n2, S.sc->n3};
+        const int s = pp.fi / 2, t = pp.fo / 2;
+        double *W = dwork + sub_base[pp.sub];
+        size_t toff[3], ttoff[3];
+        {
+          size_t o = 0;
+          for (int a = 0; a < 3; ++a) {
+            toff[a] = o; o += (size_t)n[a] * n[a] * n[a];
+            ttoff[a] = o; o += (size_t)n[a] * n[a] * n[a];
+          }
+        }
+        InflProblem p;
+        std::memset(&p, 0, sizeof(p));
+        if (s == t) {
+          const int b = s == 0 ? 1 : 0, c = s == 2 ? 1 : 2;
+          p.A = W + toff[b]; p.lda = (long)n[b] * n[b];
+          p.B = W + pp.buf2; p.ldb = n[b];
+          p.M = n[b] * n[b]; p.N = n[c] * n[c]; p.K = n[b];
+          p.P = n[b]; p.Q = n[c]; p.r2 = n[b]; p.c1 = 1; p.c2 = n[b];
+          p.dcoef = tabs[pp.sub].dcoef[s][pp.fo % 2][pp.fi % 2];
+        } else {
+          const int u = 3 - s - t;
+          if (s < u) {
+            p.A = W + pp.buf2; p.lda = (long)n[s] * n[t];
+            p.B = W + ttoff[u]; p.ldb = n[u];
+            p.M = n[s] * n[t]; p.N = n[u] * n[u]; p.K = n[u];
+            p.P = n[s]; p.Q = n[u]; p.r2 = n[s];
+            if (t < u) { p.c1 = 1; p.c2 = n[t]; } else { p.c1 = n[u]; p.c2 = 1; }
+          } else {
+            p.A = W + toff[u]; p.lda = (long)n[u] * n[u];
+            p.B = W + pp.buf2; p.ldb = n[u];
+            p.M = n[u] * n[u]; p.N = n[s] * n[t]; p.K = n[u];
+            p.P = n[u]; p.Q = n[s]; p.r2 = n[u];
+            if (t < u) { p.c1 = n[t]; p.c2 = 1; } else { p.c1 = 1; p.c2 = n[u]; }
+          }
+          p.dcoef = 0.0;
+        }
+        p.D = S.D + S.loc_off[pp.io] + S.u * S.loc_off[pp.ii];
+        p.ldd = S.u;
+        p.tiles_n = (p.N + BN - 1) / BN;
+        p.tile0 = tiles;
+        tiles += (long)((p.M + BM - 1) / BM) * p.tiles_n;
+        hprob.push_back(p);
+      }
+      L.tiles = tiles;
+      const mf::InflTile tt = mf::infl_tile_for(hprob[L.first].K);
+      L.blocks = (int)std::max<long>(1, std::min<long>(tiles, (long)ncu * tt.blocks_per_cu));
+      launches[g].push_back(L);
+    }
+  e = hipMalloc(&dprob, std::max<size_t>(hprob.size(), 1) * sizeof(InflProblem));
+  if (e == hipSuccess) e = hipMemcpy(dtab, htab.data(), htab.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !hprob.empty())
+    e = hipMemcpy(dprob, hprob.data(), hprob.size() * sizeof(InflProblem), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return fail_dev(e, "set-up");
+  // ---- the launch sequence: phase A per subdomain, then the grouped GEMMs
+  auto record = [&](hipStream_t st) -> int {
+    for (int g = 0; g < ngroups; ++g) {
+      for (int i = 0; i < nsub; ++i) {
+        if (group_of[i] != g) continue;
+        const InflSub &S = subs[i];
+        mf_solver3d *h = S.sc;
+        const int n[3] = {h->n1, h->n2, h->n3};
+        double *W = dwork + sub_base[i];
+        const double *tb = dtab;
+        const SubTabs &tt = tabs[i];
+        size_t toff[3], ttoff[3];
+        {
+          size_t o = 0;
+          for (int a = 0; a < 3; ++a) {
+            toff[a] = o; o += (size_t)n[a] * n[a] * n[a];
+            ttoff[a] = o; o += (size_t)n[a] * n[a] * n[a];
+          }
+        }
+        for (int a = 0; a < 3; ++a) {
+          const long n3 = (long)n[a] * n[a] * n[a];
+          hipLaunchKernelGGL(mf::infl_t_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                             tb + tt.V[a], tb + tt.W[a], n[a], W + toff[a], W + ttoff[a]);
+        }
+        mf::EigArgs ea;
+        ea.l[0] = h->lam; ea.l[1] = h->lam + n[0]; ea.l[2] = h->lam + n[0] + n[1];
+        ea.sigma = sigma;
+        EpiArgs ep;
+        std::memset(&ep, 0, sizeof(ep));
+        for (const PairPlan &pp : pairs) {
+          if (pp.sub != i) continue;
+          const int s = pp.fi / 2, t = pp.fo / 2;
+          double *b1 = W + pp.buf1, *b2 = W + pp.buf2;
+          if (s == t) {
+            const int b = s == 0 ? 1 : 0, c = s == 2 ? 1 : 2;
+            const long cnt = 2L * n[b] * n[c];
+            hipLaunchKernelGGL(mf::infl_g_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                               ea, s, b, c, n[s], n[b], n[c], tb + tt.what[s][0], tb + tt.what[s][1],
+                               tb + tt.ahat[pp.fi], b1);
+            // Z = G_side TT_c
+            const double *G = b1 + (long)(pp.fo % 2) * n[b] * n[c];
+            launch_gemm<double, double, double, double, true, mf::EPI_STORE>(
+                h, G, n[b], 0, W + ttoff[c], n[c], b2, n[b], 0, n[b], n[c] * n[c], n[c], 1, ep, st);
+          } else {
+            const int u = 3 - s - t;
+            const int layout = s < u ? 0 : 1;
+            const long cnt = (long)n[0] * n[1] * n[2];
+            hipLaunchKernelGGL(mf::infl_h_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                               ea, s, t, u, n[s], n[t], n[u], layout, tb + tt.Om[t][pp.fo % 2], b1);
+            if (layout == 0)  // Y = (Vs diag(ahat)) H
+              launch_gemm<double, double, double, double, true, mf::EPI_STORE>(
+                  h, tb + tt.VsA[pp.fi], n[s], 0, b1, n[s], b2, n[s], 0, n[s], n[t] * n[u], n[s], 1,
+                  ep, st);
+            else  // Y'_ct = H'_ct (Vs diag(ahat))^T, batched over ct
+              launch_gemm<double, double, double, double, true, mf::EPI_STORE>(
+                  h, b1, n[u], (long)n[u] * n[s], tb + tt.VsT[pp.fi], n[s], b2, n[u],
+                  (long)n[u] * n[s], n[u], n[s], n[s], n[t], ep, st);
+          }
+        }
+      }
+      for (const Launch &L : launches[g])
+        launch_infl_tile(L.tile, dprob + L.first, L.nprob, L.tiles, L.blocks, st);
+      if (hipGetLastError() != hipSuccess) return mf::fail(MF_ERR_DEVICE, "influence_assemble: launch");
+    }
+    return MF_OK;
+  };
+  if (graph) {
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail_dev(e, "hipStreamBeginCapture");
+    rc = record(cs);
+    e = hipStreamEndCapture(cs, &gr);
+    if (rc != MF_OK) { cleanup(); return rc; }
+    if (e != hipSuccess) return fail_dev(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, cs);
+    if (e != hipSuccess) return fail_dev(e, "graph launch");
+  } else {
+    rc = record(cs);
+    if (rc != MF_OK) { cleanup(); return rc; }
+  }
+  e = hipStreamSynchronize(cs);
+  if (e != hipSuccess) return fail_dev(e, "hipStreamSynchronize");
+  cleanup();
+  return MF_OK;
+}
+
+// Autotuner measurement: one block product of M x N x K through the grouped
+// kernel with tile class `tile` and blocks_per_cu blocks per compute unit;
+// mean milliseconds per launch over reps launches into ms.
+extern "C" int mf_tune_infl_gemm(int tile, int blocks_per_cu, int M, int N, int K, int reps,
+                                 double *ms) {
+  if (tile < 0 || tile > 2 || blocks_per_cu < 1 || blocks_per_cu > 16 || M < 1 || N < 1 || K < 1 ||
+      reps < 1 || !ms)
+    return mf::fail(MF_ERR_ARG, "mf_tune_infl_gemm: invalid argument");
+  double *A = nullptr, *B = nullptr, *D = nullptr;
+  mf::InflProblem *dp = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto done = [&]() {
+    if (A) (void)hipFree(A);
+    if (B) (void)hipFree(B);
+    if (D) (void)hipFree(D);
+    if (dp) (void)hipFree(dp);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  hipError_t e = hipMalloc(&A, (size_t)M * K * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&B, (size_t)K * N * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&D, (size_t)M * N * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dp, sizeof(mf::InflProblem));
+  if (e == hipSuccess) e = hipMemset(A, 0, (size_t)M * K * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(B, 0, (size_t)K * N * sizeof(double));
+  const int BM = tile_bm(tile), BN = 256 * 64 / BM;
+  mf::InflProblem p;
+  std::memset(&p, 0, sizeof(p));
+  p.A = A; p.lda = M; p.B = B; p.ldb = K; p.D = D; p.ldd = M;
+  p.M = M; p.N = N; p.K = K;
+  p.P = M; p.Q = N; p.r2 = M; p.c1 = 0; p.c2 = 0;  // D[m + M n]
+  p.tiles_n = (N + BN - 1) / BN;
+  p.tile0 = 0;
+  const long tiles = (long)((M + BM - 1) / BM) * p.tiles_n;
+  int ncu = 256, dev = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) { done(); return mf::fail(MF_ERR_DEVICE, std::string("mf_tune_infl_gemm: ") + hipGetErrorString(e)); }
+  const int blocks = (int)std::max<long>(1, std::min<long>(tiles, (long)ncu * blocks_per_cu));
+  launch_infl_tile(tile, dp, 1, tiles, blocks, nullptr);  // warm-up
+  e = hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) launch_infl_tile(tile, dp, 1, tiles, blocks, nullptr);
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e == hipSuccess) e = hipGetLastError();
+  done();
+  if (e != hipSuccess) return mf::fail(MF_ERR_DEVICE, std::string("mf_tune_infl_gemm: ") + hipGetErrorString(e));
+  *ms = t / reps;
   return MF_OK;
 }
