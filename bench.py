@@ -478,10 +478,15 @@ def run_md(args):
 def run_influence(args):
     """Influence-matrix decomposition on one GPU: 4x4x2 = 32 subdomains with
     per-slab sizes drawn from {48,64,96} (seed 1), asin-stretched FP64 grids,
-    Dirichlet interfaces; every unit-impulse response of every interface face
-    (md_influence_matrix_set, class_field.f90:1396-1510) in batched launches.
-    Reports local solves/s over the whole set.  Setup (coefficients and
-    eigen-decompositions) is outside the timed region."""
+    Dirichlet interfaces, sigma of the NS velocity solve.  The influence
+    matrix (md_influence_matrix_set, class_field.f90:1396-1510: one unit-
+    impulse response per interface unknown of every subdomain) is formed by
+    spectral contraction in one HIP-graph-captured sequence whose block
+    products run as persistent grouped FP64-MFMA launches.  Reports impulse
+    responses per second of device time (each block column is one response),
+    the grouped GEMM's FP64 rate, and, on a bounded sample (the faces of the
+    largest subdomain, 4096 impulses), the rate of the per-impulse solve path
+    that forms the same responses with one solve each."""
     import numpy as np
     import torch
 
@@ -491,67 +496,74 @@ def run_influence(args):
     torch.cuda.set_device(0)
     sigma = -3000.0 * 1.5 / 1e-3
     slab, subs = synthetic.subdomain_layout()
-    chunk = 4096
-    total_imp, total_s, total_flop, ref_flop = 0, 0.0, 0.0, 0.0
-    gemm_ms, gemm_work = 0.0, 0.0
-    parts = {}
+    nd = tuple(len(v) for v in slab)
+    grids = [[None] * nd[ax] for ax in range(3)]
+    for sd in subs:
+        for ax in range(3):
+            grids[ax][sd["index"][ax]] = sd["grids"][ax]
     log("influence: slab sizes x=%s y=%s z=%s" % tuple(slab))
-    for si, sd in enumerate(subs):
-        gx, gy, gz = sd["grids"]
-        shape = (len(gx), len(gy), len(gz))
-        n1, n2, n3 = (s - 2 for s in shape)
-        sc = mf.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
-        dcs = [mf.DerivativeCoefficients(g, solver=True, so=6) for g in (gx, gy, gz)]
-        stride = 2 * (n2 * n3 + n1 * n3 + n1 * n2)
-        out = torch.empty((chunk, stride), device="cuda", dtype=torch.float64)
-        # warm the workspace once (untimed)
-        mf.influence_impulses(sc, dcs, sigma, sd["interfaces"][0], 0, 1, out=out[:1])
+    responses = 0
+    for sd in subs:
+        shape = tuple(len(g) for g in sd["grids"])
+        u = sum(mf.face_points(shape, f) for f in sd["interfaces"])
+        responses += u
+    reps = max(1, min(args.steps, 3))
+    times, walls = [], []
+    h = None
+    for _ in range(reps):
+        del h
         torch.cuda.synchronize()
-        sc.profile(True)
-        sc.profile_read(reset=True)
         t0 = time.perf_counter()
-        nimp = 0
-        for face in sd["interfaces"]:
-            npts = mf.face_points(shape, face)
-            for first in range(0, npts, chunk):
-                cnt = min(chunk, npts - first)
-                mf.influence_impulses(sc, dcs, sigma, face, first, cnt, out=out[:cnt])
-                nimp += cnt
+        h = mf.Multidomain(grids, sigma)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        prof = sc.profile_read(reset=True)
-        sc.profile(False)
-        gemm_ms += prof["transform_fp64"][0]
-        gemm_work += prof["transform_fp64"][2]
-        for k in ("fold", "transform_fp64", "faces"):
-            parts[k] = parts.get(k, 0.0) + prof[k][0]
-        total_imp += nimp
-        total_s += dt
-        per = 2.0 * n1 * n2 * n3 * (n1 + n2 + n3)
-        total_flop += prof["transform_fp64"][2]
-        ref_flop += nimp * 2 * per  # a full solve per impulse, as the reference does
-        log("subdomain %2d %s: %d impulses in %.3f s" % (si, shape, nimp, dt))
-        del sc, dcs, out
-    achieved = gemm_work / (gemm_ms * 1e-3) / 1e12 if gemm_ms else None
+        walls.append(time.perf_counter() - t0)
+        ms, flop, nbytes = h.assembly_stats()
+        times.append(ms)
+    ms = min(times)
+    log("influence: %d impulse responses, %.1f GB of blocks; device assembly %.1f ms (runs %s), "
+        "create incl. host set-up %.2f s" % (responses, nbytes / 1e9, ms, ["%.1f" % t for t in times],
+                                           min(walls)))
+    tflops = flop / (ms * 1e-3) / 1e12
+    # the per-impulse solve path on a bounded sample: the largest subdomain
+    big = max(subs, key=lambda sd: np.prod([len(g) for g in sd["grids"]]))
+    gx, gy, gz = big["grids"]
+    shape = (len(gx), len(gy), len(gz))
+    n1, n2, n3 = (q - 2 for q in shape)
+    sc = mf.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
+    dcs = [mf.DerivativeCoefficients(g, solver=True, so=6) for g in (gx, gy, gz)]
+    stride = 2 * (n2 * n3 + n1 * n3 + n1 * n2)
+    cnt = 4096
+    out = torch.empty((cnt, stride), device="cuda", dtype=torch.float64)
+    face = big["interfaces"][0]
+    mf.influence_impulses(sc, dcs, sigma, face, 0, 1, out=out[:1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mf.influence_impulses(sc, dcs, sigma, face, 0, cnt, out=out)
+    torch.cuda.synchronize()
+    t_imp = time.perf_counter() - t0
+    log("influence: per-impulse solve path on %s: %d impulses in %.3f s (%.0f responses/s)"
+        % (shape, cnt, t_imp, cnt / t_imp))
+    del sc, dcs, out
     out = {
-        "metric": "influence-matrix unit-impulse local solves/s (config 3)",
-        "value": round(total_imp / total_s, 1),
-        "unit": "local solves/s",
+        "metric": "influence-matrix unit-impulse responses/s (config 3)",
+        "value": round(responses / (ms * 1e-3), 1),
+        "unit": "responses/s",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "influence-4x4x2-{48,64,96}-seed1", "slab_sizes": slab,
                    "subdomains": len(subs), "sigma": sigma},
-        "impulses": total_imp,
-        "seconds": round(total_s, 4),
-        "gemm_tflops": round(achieved, 2) if achieved else None,
-        "gemm_frac_fp64_peak": round(achieved / PEAK["fp64_mfma_tflops"], 4) if achieved else None,
-        "flop_done": total_flop,
-        "flop_reference_algorithm": ref_flop,
-        "kernel_seconds": {"spectral_impulse": round(parts.get("fold", 0) / 1e3, 4),
-                           "backward_transforms": round(parts.get("transform_fp64", 0) / 1e3, 4),
-                           "face_derivatives": round(parts.get("faces", 0) / 1e3, 4)},
+        "responses": responses,
+        "device_ms": round(ms, 2),
+        "create_s_incl_host_setup": round(min(walls), 3),
+        "block_gb": round(nbytes / 1e9, 2),
+        "grouped_gemm_flop": flop,
+        "fp64_tflops_over_device_time": round(tflops, 2),
+        "frac_fp64_peak": round(tflops / PEAK["fp64_mfma_tflops"], 4),
+        "block_write_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+        "per_impulse_path_responses_per_s": round(cnt / t_imp, 1),
+        "per_impulse_path_sample": "%d impulses on one face of the %dx%dx%d subdomain" % ((cnt,) + shape),
     }
     print(json.dumps(out), flush=True)
 

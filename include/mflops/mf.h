@@ -235,6 +235,11 @@ int mf_md_create_ex(const int nd[3], const int *npts, const double *grids, const
  *    to zero (md_vector_zero_lastpoint :251-267; the reference does so on the
  *    pressure solve, the one caller that passes null).  Default 0. */
 int mf_md_set_null_space(mf_md *h, int psm);
+/* No counterpart in the reference: device time of the influence-block
+ * assembly of mf_md_create* (0 when the blocks came from a cache file or the
+ * unit-impulse path), the FP64 flops of its grouped block products and the
+ * bytes of blocks they wrote. */
+int mf_md_assembly_stats(const mf_md *h, double *device_ms, double *gemm_flop, double *block_bytes);
 /* No counterpart in the reference: the measurement of the autotuner
  * (mflops3d-md_amd/tune/infl_autotune.py) that chooses the tile shape of the
  * grouped influence-block GEMM: one M x N x K block product through that

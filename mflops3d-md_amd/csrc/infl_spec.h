@@ -126,9 +126,9 @@ __global__ void __launch_bounds__(256) infl_h_kernel(EigArgs e, int s, int t_ax,
 
 // One block product of the grouped launch: C = A B (A: M x K column-major,
 // B: K x N with K contiguous), written into the subdomain matrix at
-//   row = m1 + r2 n1,  col = c1 m2 + c2 n2,  m = m1 + P m2,  n = n1 + Q n2,
-// plus dcoef where m1 == m2 and n1 == n2 (the impulse's own boundary value
-// on its line, s = t blocks only).
+//   row = m1 + r2 n1,  col = c1 m2 + c2 n2,  m = m1 + P m2,  n = n1 + Q n2.
+// For s = t blocks the impulse's own boundary value on its line adds dcoef
+// on the block diagonal (infl_diag_kernel, after the products).
 struct InflProblem {
   const double *A;
   const double *B;
@@ -141,11 +141,33 @@ struct InflProblem {
   double dcoef;
 };
 
+// D[q + ldd q] += dcoef for q < count (the diagonal of an s = t block);
+// grid (ceil(max count / 256), blocks)
+struct InflDiag {
+  double *D;
+  long ldd;
+  int count;
+  double dcoef;
+};
+__global__ void __launch_bounds__(256) infl_diag_kernel(const InflDiag *__restrict__ dg) {
+  const InflDiag g = dg[blockIdx.y];
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= g.count) return;
+  g.D[q + g.ldd * q] += g.dcoef;
+}
+
+// Pointers read from the problem list are generic to the compiler; these
+// casts let it emit global (not flat) memory instructions.
+typedef const double __attribute__((address_space(1))) gcdouble;
+typedef double __attribute__((address_space(1))) gdouble;
+
 // Persistent grouped FP64 MFMA GEMM over a list of block products: each
 // block walks the global tile list with stride gridDim.x, locates its
 // problem by binary search over tile0 and computes one BM x BN tile (four
 // 64 x 64 wave tiles, k-steps of 16 staged through LDS).  Every tile
 // writes distinct matrix entries, so no ordering between blocks is needed.
+// The matrix is written with non-temporal stores (it is far larger than
+// the caches and read only by later solves).
 template <int BM, int BN>
 __global__ void __launch_bounds__(256) infl_gemm_kernel(const InflProblem *__restrict__ probs,
                                                         int nprob, long total_tiles) {
@@ -168,37 +190,32 @@ __global__ void __launch_bounds__(256) infl_gemm_kernel(const InflProblem *__res
       const int mid = (lo + hi + 1) >> 1;
       if (probs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
     }
-    const InflProblem &p = probs[lo];
-    const double *__restrict__ A = p.A;
-    const double *__restrict__ B = p.B;
-    const long lda = p.lda, ldb = p.ldb;
+    const InflProblem p = probs[lo];
+    gcdouble *A = (gcdouble *)p.A;
+    gcdouble *B = (gcdouble *)p.B;
     const int M = p.M, N = p.N, K = p.K;
+    const int lda = (int)p.lda, ldb = (int)p.ldb;
     const long lt = tile - p.tile0;
-    const long m0 = (lt / p.tiles_n) * BM;
-    const long n0 = (lt % p.tiles_n) * BN;
+    const int m0 = (int)(lt / p.tiles_n) * BM;
+    const int n0 = (int)(lt % p.tiles_n) * BN;
     d4_t acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0, 0, 0, 0};
     double ra[NA], rb[NB];
-    auto load = [&](int kt) {
 #pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        const int e = tid + 256 * q;
-        const long m = m0 + (e % BM);
-        const int k = kt + e / BM;
-        ra[q] = (m < M && k < K) ? A[m + lda * k] : 0.0;
-      }
+    for (int q = 0; q < NA; ++q) {
+      const int e = tid + 256 * q;
+      const int m = m0 + (e % BM), k = e / BM;
+      ra[q] = (m < M && k < K) ? A[m + (long)lda * k] : 0.0;
+    }
 #pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        const int e = tid + 256 * q;
-        const int k = kt + e % BK;
-        const long n = n0 + e / BK;
-        rb[q] = (n < N && k < K) ? B[k + ldb * n] : 0.0;
-      }
-    };
-    load(0);
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + 256 * q;
+      const int k = e % BK, n = n0 + e / BK;
+      rb[q] = (n < N && k < K) ? B[k + (long)ldb * n] : 0.0;
+    }
     for (int kt = 0; kt < K; kt += BK) {
       __syncthreads();
 #pragma unroll
@@ -212,7 +229,20 @@ __global__ void __launch_bounds__(256) infl_gemm_kernel(const InflProblem *__res
         Bs[e / BK][e % BK] = rb[q];
       }
       __syncthreads();
-      if (kt + BK < K) load(kt + BK);
+      if (kt + BK < K) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+          const int e = tid + 256 * q;
+          const int m = m0 + (e % BM), k = kt + BK + e / BM;
+          ra[q] = (m < M && k < K) ? A[m + (long)lda * k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int e = tid + 256 * q;
+          const int k = kt + BK + e % BK, n = n0 + e / BK;
+          rb[q] = (n < N && k < K) ? B[k + (long)ldb * n] : 0.0;
+        }
+      }
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         double af[4], bf[4];
@@ -228,26 +258,33 @@ __global__ void __launch_bounds__(256) infl_gemm_kernel(const InflProblem *__res
             acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[ni], af[mi], acc[mi][ni], 0, 0, 0);
       }
     }
-    // epilogue: lane holds C(m0+wm+16mi+lane%16, n0+wn+16ni+lane/16+4r)
-    double *__restrict__ D = p.D;
+    // epilogue: lane holds C(mb + 16 mi, nb + 16 ni + 4 r), mb = m0+wm+lane%16,
+    // nb = n0+wn+lane/16; the (m1, m2) and (n1, n2) splits advance with the
+    // offsets (one division each per tile)
+    gdouble *D = (gdouble *)p.D;
     const long ldd = p.ldd;
+    const int P = p.P, Q = p.Q;
+    const int nb = n0 + wn + (lane >> 4);
+    const int q1b = nb % Q, q2b = nb / Q;
+    const int mb = m0 + wm + (lane & 15);
+    int m1 = mb % P, m2 = mb / P;
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
-      const long m = m0 + wm + mi * 16 + (lane & 15);
-      if (m >= M) continue;
-      const int m1 = (int)(m % p.P), m2 = (int)(m / p.P);
+      if (mi > 0) {
+        m1 += 16;
+        while (m1 >= P) { m1 -= P; ++m2; }
+      }
+      if (mb + 16 * mi >= M) continue;
       const long colm = (long)p.c1 * m2;
+      int a1 = q1b, a2 = q2b;  // column split of nb + 4 d, advanced with d
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const long n = n0 + wn + ni * 16 + (lane >> 4) + 4 * r;
-          if (n >= N) continue;
-          const int q1 = (int)(n % p.Q), q2 = (int)(n / p.Q);
-          double v = acc[mi][ni][r];
-          if (m1 == m2 && q1 == q2) v += p.dcoef;
-          D[(m1 + (long)p.r2 * q1) + ldd * (colm + (long)p.c2 * q2)] = v;
-        }
+      for (int d = 0; d < 16; ++d) {
+        if (nb + 4 * d < N)
+          __builtin_nontemporal_store(
+              acc[mi][d >> 2][d & 3], D + ((m1 + (long)p.r2 * a1) + ldd * (colm + (long)p.c2 * a2)));
+        a1 += 4;
+        while (a1 >= Q) { a1 -= Q; ++a2; }
+      }
     }
     __syncthreads();  // LDS reuse by the next tile
   }

@@ -58,7 +58,13 @@ struct InflSub {
   long u;
   double *D;
 };
-int influence_assemble(const std::vector<InflSub> &subs, double sigma, bool graph, void *stream);
+struct InflStats {
+  double ms = 0.0;         // device time of the whole sequence (events around it)
+  double gemm_flop = 0.0;  // 2 M N K of the grouped block products
+  double block_bytes = 0.0;  // bytes of the blocks written
+};
+int influence_assemble(const std::vector<InflSub> &subs, double sigma, bool graph, void *stream,
+                       InflStats *stats = nullptr);
 
 // host copies of a derivative handle's tables (order 1|2, sub 0|1)
 bool der_host_tables(const struct ::mf_der *dc, int order, int sub, const double **dl_lu,

@@ -276,6 +276,7 @@ struct mf_md {
   // entry zeroed (md_vector_zero_lastpoint)
   int psm = 0;
   double *ones = nullptr;  // ninf ones, for psm = 1
+  mf::InflStats asm_stats;  // the device assembly of the blocks
   std::vector<Sub> subs;
   std::vector<long> ioff;  // interface offsets (size it+1)
   long ninf = 0, nloc = 0;
@@ -612,7 +613,7 @@ int md_create(const int nd[3], const int *npts, const double *grids, const int b
     }
     const char *g_env = std::getenv("MF_INFL_GRAPH");
     const bool graph = !(g_env && g_env[0] == '0');
-    const int rc = mf::influence_assemble(is, sigma, graph, nullptr);
+    const int rc = mf::influence_assemble(is, sigma, graph, nullptr, &h->asm_stats);
     if (rc != MF_OK) return bail(rc);
     *out = h;
     return MF_OK;
@@ -671,6 +672,15 @@ int md_create(const int nd[3], const int *npts, const double *grids, const int b
 extern "C" int mf_md_create(const int nd[3], const int *npts, const double *grids, const int bct[6],
                             int so, double sigma, mf_md **out) {
   return md_create(nd, npts, grids, bct, nullptr, so, sigma, true, out);
+}
+
+extern "C" int mf_md_assembly_stats(const mf_md *h, double *device_ms, double *gemm_flop,
+                                    double *block_bytes) {
+  if (!h) return mf::fail(MF_ERR_ARG, "mf_md_assembly_stats: null handle");
+  if (device_ms) *device_ms = h->asm_stats.ms;
+  if (gemm_flop) *gemm_flop = h->asm_stats.gemm_flop;
+  if (block_bytes) *block_bytes = h->asm_stats.block_bytes;
+  return MF_OK;
 }
 
 extern "C" int mf_md_set_null_space(mf_md *h, int psm) {

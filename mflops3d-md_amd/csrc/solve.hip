@@ -1468,7 +1468,8 @@ void launch_infl_tile(int tile, const mf::InflProblem *probs, int nprob, long ti
 
 }  // namespace
 
-int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool graph, void *stream) {
+int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool graph, void *stream,
+                           InflStats *stats) {
   using mf::InflProblem;
   if (subs.empty()) return MF_OK;
   const int nsub = (int)subs.size();
@@ -1585,20 +1586,26 @@ int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool 
   // ---- device buffers
   double *dtab = nullptr, *dwork = nullptr;
   InflProblem *dprob = nullptr;
+  mf::InflDiag *ddiag = nullptr;
   struct Launch { int tile; long first; int nprob; long tiles; int blocks; };
   std::vector<std::vector<Launch>> launches(ngroups);
   std::vector<InflProblem> hprob;
+  std::vector<std::vector<mf::InflDiag>> hdiag(ngroups);
   int rc = MF_OK;
   hipStream_t cs = nullptr;
   hipGraph_t gr = nullptr;
   hipGraphExec_t ge = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   auto cleanup = [&]() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     if (ge) (void)hipGraphExecDestroy(ge);
     if (gr) (void)hipGraphDestroy(gr);
     if (cs) { (void)hipStreamSynchronize(cs); (void)hipStreamDestroy(cs); }
     if (dtab) (void)hipFree(dtab);
     if (dwork) (void)hipFree(dwork);
     if (dprob) (void)hipFree(dprob);
+    if (ddiag) (void)hipFree(ddiag);
   };
   auto fail_dev = [&](hipError_t e, const char *what) {
     cleanup();
@@ -1659,6 +1666,14 @@ int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool 
         }
         p.D = S.D + S.loc_off[pp.io] + S.u * S.loc_off[pp.ii];
         p.ldd = S.u;
+        if (s == t) {
+          mf::InflDiag dg;
+          dg.D = p.D;
+          dg.ldd = p.ldd;
+          dg.count = p.P * p.Q;
+          dg.dcoef = p.dcoef;
+          hdiag[g].push_back(dg);
+        }
         p.tiles_n = (p.N + BN - 1) / BN;
         p.tile0 = tiles;
         tiles += (long)((p.M + BM - 1) / BM) * p.tiles_n;
@@ -1669,13 +1684,39 @@ int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool 
       L.blocks = (int)std::max<long>(1, std::min<long>(tiles, (long)ncu * tt.blocks_per_cu));
       launches[g].push_back(L);
     }
+  // diagonal terms of every group, contiguous
+  std::vector<mf::InflDiag> alld;
+  std::vector<size_t> diag_first(ngroups + 1, 0);
+  int diag_max = 1;
+  for (int g = 0; g < ngroups; ++g) {
+    diag_first[g] = alld.size();
+    for (const mf::InflDiag &d : hdiag[g]) {
+      alld.push_back(d);
+      diag_max = std::max(diag_max, d.count);
+    }
+  }
+  diag_first[ngroups] = alld.size();
+  e = hipMalloc(&ddiag, std::max<size_t>(alld.size(), 1) * sizeof(mf::InflDiag));
+  if (e == hipSuccess && !alld.empty())
+    e = hipMemcpy(ddiag, alld.data(), alld.size() * sizeof(mf::InflDiag), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail_dev(e, "diagonal list");
   e = hipMalloc(&dprob, std::max<size_t>(hprob.size(), 1) * sizeof(InflProblem));
   if (e == hipSuccess) e = hipMemcpy(dtab, htab.data(), htab.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess && !hprob.empty())
     e = hipMemcpy(dprob, hprob.data(), hprob.size() * sizeof(InflProblem), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
   if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return fail_dev(e, "set-up");
+  if (stats) {
+    stats->gemm_flop = 0.0;
+    stats->block_bytes = 0.0;
+    for (const InflProblem &p : hprob) {
+      stats->gemm_flop += 2.0 * p.M * p.N * p.K;
+      stats->block_bytes += 8.0 * p.M * p.N;
+    }
+  }
   // ---- the launch sequence: phase A per subdomain, then the grouped GEMMs
   auto record = [&](hipStream_t st) -> int {
     for (int g = 0; g < ngroups; ++g) {
@@ -1738,6 +1779,10 @@ int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool 
       }
       for (const Launch &L : launches[g])
         launch_infl_tile(L.tile, dprob + L.first, L.nprob, L.tiles, L.blocks, st);
+      const int ndg = (int)(diag_first[g + 1] - diag_first[g]);
+      if (ndg > 0)
+        hipLaunchKernelGGL(mf::infl_diag_kernel, dim3((unsigned)((diag_max + 255) / 256), ndg), dim3(256), 0,
+                           st, (const mf::InflDiag *)(ddiag + diag_first[g]));
       if (hipGetLastError() != hipSuccess) return mf::fail(MF_ERR_DEVICE, "influence_assemble: launch");
     }
     return MF_OK;
@@ -1750,14 +1795,24 @@ int mf::influence_assemble(const std::vector<InflSub> &subs, double sigma, bool 
     if (rc != MF_OK) { cleanup(); return rc; }
     if (e != hipSuccess) return fail_dev(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventRecord(ev0, cs);
     if (e == hipSuccess) e = hipGraphLaunch(ge, cs);
+    if (e == hipSuccess) e = hipEventRecord(ev1, cs);
     if (e != hipSuccess) return fail_dev(e, "graph launch");
   } else {
+    e = hipEventRecord(ev0, cs);
+    if (e != hipSuccess) return fail_dev(e, "hipEventRecord");
     rc = record(cs);
     if (rc != MF_OK) { cleanup(); return rc; }
+    e = hipEventRecord(ev1, cs);
+    if (e != hipSuccess) return fail_dev(e, "hipEventRecord");
   }
   e = hipStreamSynchronize(cs);
   if (e != hipSuccess) return fail_dev(e, "hipStreamSynchronize");
+  if (stats) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, ev0, ev1) == hipSuccess) stats->ms = t;
+  }
   cleanup();
   return MF_OK;
 }

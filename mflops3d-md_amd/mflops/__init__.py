@@ -538,6 +538,12 @@ class Multidomain:
     def set_guess(self, enable: bool = True):
         check(self._lib.mf_md_set_guess(self._h, int(bool(enable))))
 
+    def assembly_stats(self):
+        """(device ms, GEMM flops, block bytes) of the influence-block assembly."""
+        ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+        check(self._lib.mf_md_assembly_stats(self._h, C.byref(ms), C.byref(fl), C.byref(by)))
+        return ms.value, fl.value, by.value
+
     def set_null_space(self, psm: int):
         """Pressure singular method psm (0, 1, 2) of the following solves."""
         check(self._lib.mf_md_set_null_space(self._h, int(psm)))

@@ -54,6 +54,7 @@ SIGNATURES = {
     "mf_md_create_ex": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_char_p,
                                   _vp, C.POINTER(_vp)]),
     "mf_md_set_null_space": (C.c_int, [_vp, C.c_int]),
+    "mf_md_assembly_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mf_tune_infl_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.POINTER(C.c_double)]),
     "mf_solver3d_profile": (C.c_int, [_vp, C.c_int]),
