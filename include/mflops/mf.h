@@ -396,7 +396,8 @@ typedef struct mf_filter mf_filter;
  *    xo(nout) (deriv 0: splev, the reference's yo) or differentiated there
  *    (deriv 1: splder, its ypo); nout <= 256.  The fit is linear in the
  *    values: the handle holds F (nout x nin).  xi and xn increase,
- *    xn(1) = xi(1) and xn(nn) = xi(nin); MF_ERR_SETUP when a spline has no
+ *    xn(1) = xi(1) and xn(nn) = xi(nin) (to rounding: 16 ulps of the
+ *    interval; the knots take xi's end points); MF_ERR_SETUP when a spline has no
  *    data point in its support (curfit's Schoenberg-Whitney condition). */
 int mf_filter_create(const double *xi, const double *wi, int nin, const double *xo, int nout,
                      const double *xn, int nn, int k, int deriv, mf_filter **out);
@@ -476,6 +477,15 @@ int mf_sgs_features(const mf_der *dx, const mf_der *dy, const mf_der *dz, const 
                     const double *v, const double *w, int nx, int ny, int nz, const double *gx,
                     const double *gy, const double *gz, double gscale, double lscale, float *X,
                     double *work, void *stream);
+/* No counterpart in the reference: a-posteriori-consistent training targets
+ * for the learned closure.  For feature rows X (n x 10, as mf_sgs_features
+ * with gscale = lscale = 1) the Smagorinsky stress of constant c,
+ * T = -2 c^2 Delta^2 |S| S (n x 6: xx, yy, zz, xy, xz, yz), S the strain
+ * rate of the nine gradients, |S| = sqrt(2 S:S); with c fitted so that
+ * coarse runs follow the filtered fine run, and the closure coupled as an
+ * eddy viscosity (mf_ns_set_sgs_coupling 1), whose projection of this
+ * stress is exactly nu = c^2 Delta^2 |S|. */
+int mf_sgs_eddy_targets(const float *X, long long n, double c, float *T, void *stream);
 /* The learned closure in the NS step (tau enters the momentum equation as
  * -div tau, as the eddy-viscosity stress does in f(nav,var) :1083-1134):
  * each step takes X of the latest level as mf_sgs_features does, tau =
@@ -484,6 +494,14 @@ int mf_sgs_features(const mf_der *dx, const mf_der *dy, const mf_der *dz, const 
  * off; the handle must outlive its use. */
 int mf_ns_set_sgs(mf_ns *ns, const mf_mlp *mlp, double gscale, double lscale, double tscale,
                   void *stream);
+/* No counterpart in the reference (its closures are eddy viscosities): how
+ * the learned stress enters the momentum force.  0 (default): -div tau.
+ * 1: as the eddy viscosity nu = max(0, -tau:S) / (2 S:S) of the strain rate
+ * S of the same gradients (the stress projected on the dissipative
+ * direction, backscatter clipped), added as the LES force is; the velocity
+ * stays bounded where the raw stress's divergence is noisy at the grid
+ * scale.  mf_ns_les_nu then returns this nu. */
+int mf_ns_set_sgs_coupling(mf_ns *ns, int coupling);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,10 @@ void sgs_features_launch(const double *const du[9], const double *d2x, const dou
                          float *X, void *stream);
 void sgs_aux_launch(const float *Y, int j, double tscale, double *o0, double *o1, double *o2,
                     int nx, int ny, int nz, void *stream);
+// nu = max(0, -(tau : S)) / (2 S : S) of the closure's outputs (tau = tscale Y)
+// and the gradients du (du[3 i + j] = d u_i / d x_j); 0 on edges and where S = 0
+void sgs_nu_launch(const float *Y, const double *const du[9], double tscale, double *nu, int nx,
+                   int ny, int nz, void *stream);
 
 // NetCDF classic field files (io.cpp): one 3-D variable f(nx,ny,nz) of a
 // host array, FP64 (dbl) or FP32 on disk; reading converts to FP64.

@@ -357,6 +357,13 @@ __device__ __forceinline__ int pair_col(int i, int j) {
 }
 
 // o_i = -tscale Y[t][pair(i, j)] for one direction j, zero on the box edges
+struct SgsNuArgs {
+  const double *du[9];
+  double tscale;
+  double *nu;
+  int nx, ny, nz;
+};
+
 __global__ void __launch_bounds__(256) sgs_aux_kernel(const float *__restrict__ Y, int jd,
                                                       double tscale, double *__restrict__ o0,
                                                       double *__restrict__ o1,
@@ -374,8 +381,70 @@ __global__ void __launch_bounds__(256) sgs_aux_kernel(const float *__restrict__ 
   o2[t] = s * (double)y[pair_col(2, jd)];
 }
 
+// Eddy viscosity of the learned stress projected on the strain rate:
+// nu = max(0, -(tau : S)) / (2 S : S), tau = tscale Y (xx yy zz xy xz yz),
+// S_ij = (du_ij + du_ji)/2; 0 on the box edges and where S : S = 0.
+__global__ void __launch_bounds__(256) sgs_nu_kernel(const float *__restrict__ Y, SgsNuArgs a) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long N = (long)a.nx * a.ny * a.nz;
+  if (t >= N) return;
+  const int i = (int)(t % a.nx), j = (int)((t / a.nx) % a.ny), k = (int)(t / ((long)a.nx * a.ny));
+  const int nb = (i == 0 || i == a.nx - 1) + (j == 0 || j == a.ny - 1) + (k == 0 || k == a.nz - 1);
+  double g[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) g[q] = a.du[q][t];
+  const double sxx = g[0], syy = g[4], szz = g[8];
+  const double sxy = 0.5 * (g[1] + g[3]), sxz = 0.5 * (g[2] + g[6]), syz = 0.5 * (g[5] + g[7]);
+  const float *y = Y + t * O;
+  const double ts = a.tscale;
+  const double tS = ts * ((double)y[0] * sxx + (double)y[1] * syy + (double)y[2] * szz +
+                          2.0 * ((double)y[3] * sxy + (double)y[4] * sxz + (double)y[5] * syz));
+  const double SS = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+  a.nu[t] = (nb >= 2 || !(SS > 0.0)) ? 0.0 : fmax(0.0, -tS) / (2.0 * SS);
+}
+
+// Eddy-viscosity stress targets of feature rows: tau = -2 c^2 Delta^2 |S| S
+// (xx yy zz xy xz yz), S from the nine gradients (row-major ij), Delta the
+// tenth column, |S| = sqrt(2 S:S): the Smagorinsky stress of constant c
+__global__ void __launch_bounds__(256) eddy_targets_kernel(const float *__restrict__ X, long n,
+                                                           double c2, float *__restrict__ T) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float *x = X + t * I0;
+  const double sxx = x[0], syy = x[4], szz = x[8];
+  const double sxy = 0.5 * ((double)x[1] + (double)x[3]), sxz = 0.5 * ((double)x[2] + (double)x[6]),
+               syz = 0.5 * ((double)x[5] + (double)x[7]);
+  const double SS = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+  const double d = x[9];
+  const double f = -2.0 * c2 * d * d * sqrt(2.0 * SS);
+  float *o = T + t * O;
+  o[0] = (float)(f * sxx); o[1] = (float)(f * syy); o[2] = (float)(f * szz);
+  o[3] = (float)(f * sxy); o[4] = (float)(f * sxz); o[5] = (float)(f * syz);
+}
+
 }  // namespace les
 }  // namespace mf
+
+extern "C" int mf_sgs_eddy_targets(const float *X, long long n, double c, float *T, void *stream) {
+  if (!X || !T || n < 0 || !(c >= 0.0)) return mf::fail(MF_ERR_ARG, "mf_sgs_eddy_targets: invalid argument");
+  if (n == 0) return MF_OK;
+  hipLaunchKernelGGL(mf::les::eddy_targets_kernel, dim3(mf::les::nblk(n)), dim3(256), 0,
+                     (hipStream_t)stream, X, (long)n, c * c, T);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mf::fail(MF_ERR_DEVICE, std::string("mf_sgs_eddy_targets: ") + hipGetErrorString(e));
+  return MF_OK;
+}
+
+void mf::sgs_nu_launch(const float *Y, const double *const du[9], double tscale, double *nu, int nx,
+                       int ny, int nz, void *stream) {
+  mf::les::SgsNuArgs a;
+  for (int q = 0; q < 9; ++q) a.du[q] = du[q];
+  a.tscale = tscale;
+  a.nu = nu;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  hipLaunchKernelGGL(mf::les::sgs_nu_kernel, dim3(mf::les::nblk((long)nx * ny * nz)), dim3(256), 0,
+                     (hipStream_t)stream, Y, a);
+}
 
 void mf::sgs_features_launch(const double *const du[9], const double *d2x, const double *d2y,
                              const double *d2z, int nx, int ny, int nz, double gscale,
@@ -465,8 +534,14 @@ extern "C" int mf_filter_create(const double *xi, const double *wi, int nin, con
   if (wi)
     for (int r = 0; r < nin; ++r)
       if (!(wi[r] > 0)) return mf::fail(MF_ERR_ARG, "mf_filter_create: weights must be positive");
-  if (xn[0] != xi[0] || xn[nn - 1] != xi[nin - 1])
-    return mf::fail(MF_ERR_ARG, "mf_filter_create: the nodes must start and end at the input end points");
+  // the end nodes are the input end points (the knots below use xi's); two
+  // grids built independently may differ there by rounding
+  {
+    const double span = std::fabs(xi[nin - 1]) + std::fabs(xi[0]) + (xi[nin - 1] - xi[0]);
+    const double tol = 16.0 * 2.220446049250313e-16 * span;
+    if (std::fabs(xn[0] - xi[0]) > tol || std::fabs(xn[nn - 1] - xi[nin - 1]) > tol)
+      return mf::fail(MF_ERR_ARG, "mf_filter_create: the nodes must start and end at the input end points");
+  }
   // knots (class_filter.f90:149-154)
   std::vector<long double> t;
   for (int q = 0; q <= k; ++q) t.push_back(xi[0]);

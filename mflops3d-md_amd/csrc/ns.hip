@@ -590,6 +590,7 @@ struct mf_ns {
   // learned closure: tau = tscale MLP(gscale du, lscale Delta)
   const mf_mlp *sgs = nullptr;
   double sgs_g = 1.0, sgs_l = 1.0, sgs_t = 1.0;
+  int sgs_coupling = 0;  // 0: -div tau; 1: eddy viscosity max(0, -tau:S)/(2 S:S)
   void *sgsmem = nullptr;
   float *sgsX = nullptr, *sgsY = nullptr;
   // projection type and sub-iterations (navier_initialization :2133,2136):
@@ -709,7 +710,22 @@ int les_force(mf_ns *s, int L, hipStream_t st) {
       first = false;
     }
   }
-  if (s->sgs) {  // -div tau of the learned closure
+  if (s->sgs && s->sgs_coupling == 1) {  // the learned stress as an eddy viscosity
+    mf::sgs_features_launch(du, s->d2, s->d2 + s->nx, s->d2 + s->nx + s->ny, s->nx, s->ny, s->nz,
+                            s->sgs_g, s->sgs_l, s->sgsX, st);
+    RC(mf_mlp_forward(s->sgs, s->sgsX, s->sgsY, N, st));
+    mf::sgs_nu_launch(s->sgsY, du, s->sgs_t, s->lesnu, s->nx, s->ny, s->nz, st);
+    for (int j = 0; j < 3; ++j) {
+      hipLaunchKernelGGL(les_aux_kernel, dim3(nblk(N)), dim3(256), 0, st,
+                         (const double *)s->lesnu, du[j], du[3 + j], du[6 + j], du[3 * j],
+                         du[3 * j + 1], du[3 * j + 2], T[9], T[10], T[11], N);
+      RC(der3(s, j, T[9], T[10], T[11], T[12], T[13], T[14], st));
+      hipLaunchKernelGGL(les_acc_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[12],
+                         (const double *)T[13], (const double *)T[14], s->lesf[0], s->lesf[1],
+                         s->lesf[2], (int)first, N);
+      first = false;
+    }
+  } else if (s->sgs) {  // -div tau of the learned closure
     mf::sgs_features_launch(du, s->d2, s->d2 + s->nx, s->d2 + s->nx + s->ny, s->nx, s->ny, s->nz,
                             s->sgs_g, s->sgs_l, s->sgsX, st);
     RC(mf_mlp_forward(s->sgs, s->sgsX, s->sgsY, N, st));
@@ -1251,6 +1267,13 @@ int clear_presolve(mf_ns *s, hipStream_t st) {
   return MF_OK;
 }
 }  // namespace
+
+extern "C" int mf_ns_set_sgs_coupling(mf_ns *s, int coupling) {
+  if (!s || coupling < 0 || coupling > 1)
+    return mf::fail(MF_ERR_ARG, "mf_ns_set_sgs_coupling: coupling must be 0 or 1");
+  s->sgs_coupling = coupling;
+  return MF_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Restart files (restart_write / restart_read, class_navier_3D.f90:186-340)

@@ -470,6 +470,11 @@ class NavierStokes:
         check(self._lib.mf_ns_set_sgs(self._h, h, float(gscale), float(lscale), float(tscale),
                                       _stream(stream)))
 
+    def set_sgs_coupling(self, coupling: int):
+        """0: the closure's stress enters as -div tau; 1: as the eddy viscosity
+        max(0, -tau:S)/(2 S:S)."""
+        check(self._lib.mf_ns_set_sgs_coupling(self._h, int(coupling)))
+
     def les_nu(self, stream=None):
         out = fortran_empty(self.shape)
         check(self._lib.mf_ns_les_nu(self._h, C.c_void_p(_ptr(out)), _stream(stream)))
@@ -727,6 +732,18 @@ def sgs_targets(fx: SplineFilter, fy: SplineFilter, fz: SplineFilter, u, v, w,
                                  *[C.c_void_p(_ptr(t)) for t in bars], C.c_void_p(tau.data_ptr()),
                                  float(tscale), C.c_void_p(work.data_ptr()), _stream(stream)))
     return bars, tau
+
+
+def sgs_eddy_targets(X, c: float, stream=None):
+    """Smagorinsky stress -2 c^2 Delta^2 |S| S (n x 6 float32) of unscaled
+    feature rows X (n x 10 float32)."""
+    torch = _torch()
+    if X.dtype != torch.float32 or X.dim() != 2 or X.shape[1] != 10 or not X.is_contiguous():
+        raise ValueError("X must be a contiguous (n, 10) float32 tensor")
+    T = torch.empty((X.shape[0], 6), dtype=torch.float32, device=X.device)
+    check(load().mf_sgs_eddy_targets(C.c_void_p(X.data_ptr()), int(X.shape[0]), float(c),
+                                     C.c_void_p(T.data_ptr()), _stream(stream)))
+    return T
 
 
 def sgs_features(dcs, u, v, w, grids, gscale: float = 1.0, lscale: float = 1.0, stream=None):

@@ -203,6 +203,17 @@ def sgs_targets(Fx, Fy, Fz, u, v, w, tscale: float = 1.0):
     return qb, tau
 
 
+def eddy_targets(X, c: float):
+    """Smagorinsky stress -2 c^2 Delta^2 |S| S (n x 6: xx yy zz xy xz yz) of
+    unscaled feature rows (nine gradients row-major ij, then Delta)."""
+    X = np.asarray(X, dtype=np.float64)
+    g = [[X[:, 3 * i + j] for j in range(3)] for i in range(3)]
+    S = [[0.5 * (g[i][j] + g[j][i]) for j in range(3)] for i in range(3)]
+    SS = sum(S[i][j] ** 2 for i in range(3) for j in range(3))
+    f = -2.0 * c * c * X[:, 9] ** 2 * np.sqrt(2.0 * SS)
+    return np.stack([f * S[0][0], f * S[1][1], f * S[2][2], f * S[0][1], f * S[0][2], f * S[1][2]], 1)
+
+
 def sgs_features(du, gx, gy, gz, gscale: float = 1.0, lscale: float = 1.0):
     """X (n x 10): gscale du_ij (row-major ij, edges zero), lscale Delta."""
     cols = [gscale * zero_edges(np.asarray(du[i][j], dtype=np.float64)).ravel(order="F")

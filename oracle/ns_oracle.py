@@ -117,6 +117,21 @@ def integral_weights(gx, gy, gz) -> np.ndarray:
     return np.asfortranarray(W)
 
 
+def sgs_eddy_viscosity(tau, du):
+    """nu = max(0, -tau:S) / (2 S:S), S_ij = (du_ij + du_ji)/2, tau the six
+    columns (xx, yy, zz, xy, xz, yz); 0 on the box edges and where S = 0."""
+    sxx, syy, szz = du[0][0], du[1][1], du[2][2]
+    sxy = 0.5 * (du[0][1] + du[1][0])
+    sxz = 0.5 * (du[0][2] + du[2][0])
+    syz = 0.5 * (du[1][2] + du[2][1])
+    tS = (tau[0] * sxx + tau[1] * syy + tau[2] * szz
+          + 2.0 * (tau[3] * sxy + tau[4] * sxz + tau[5] * syz))
+    SS = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nu = np.where(SS > 0.0, np.maximum(0.0, -tS) / (2.0 * SS), 0.0)
+    return zero_edges(np.asfortranarray(nu))
+
+
 class FlowRate:
     """Constant-flow-rate forcing of f(nav,'rhsu') (class_navier_3D.f90:1124-1164):
     q = integral(extrapol_v(u)) / integral(1); on the first call
@@ -179,10 +194,12 @@ class NSOracle:
         # use it to perturb the step and measure the result's sensitivity)
         self.velocity_hook = None
 
-    def set_sgs(self, params, gscale: float, lscale: float, tscale: float):
+    def set_sgs(self, params, gscale: float, lscale: float, tscale: float, coupling: int = 0):
         """Learned closure: tau = tscale MLP(X), X as les.sgs_features (rounded
-        to float32, as the device stores it), added as -div tau."""
+        to float32, as the device stores it), added as -div tau (coupling 0)
+        or as the eddy viscosity nu = max(0, -tau:S)/(2 S:S) (coupling 1)."""
         self.sgs = (np.asarray(params, dtype=np.float32), gscale, lscale, tscale)
+        self.sgs_coupling = coupling
 
     def set_les(self, les_type: int, c: float):
         """les_type 0..3 and the command-line constant (squared, :2138)."""
@@ -205,9 +222,16 @@ class NSOracle:
             X = les.sgs_features(du, *self.grids, gs, ls).astype(np.float32)
             Y = les.forward(p, X)[0]
             tau = [ts * Y[:, k].reshape(self.shape, order="F") for k in range(6)]
-            f = self.minus_div_tau(tau)
-            for c in "uvw":
-                out[c] = out[c] + f[c]
+            if getattr(self, "sgs_coupling", 0) == 1:
+                nu = sgs_eddy_viscosity(tau, du)
+                self.les_nu = nu
+                for i, c in enumerate("uvw"):
+                    for j in range(3):
+                        out[c] = out[c] + self.der(nu * (du[i][j] + du[j][i]), j)
+            else:
+                f = self.minus_div_tau(tau)
+                for c in "uvw":
+                    out[c] = out[c] + f[c]
         return out
 
     def minus_div_tau(self, tau):
