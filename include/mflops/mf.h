@@ -103,6 +103,12 @@ int mf_der3d(const mf_der *dc, int axis, int order, const double *x, double *out
              int nx, int ny, int nz, void *stream);
 int mf_der3d_host(const mf_der *dc, int axis, int order, const double *x, double *out,
                   int nx, int ny, int nz);
+/* mf_der_lines on host arrays (the Fortran der/dder/der_s/dder_s of one
+ * line, class_derivatives.f90:186-260).  The host-array entry points stage
+ * through a device buffer kept in the handle (grown to the largest call),
+ * so repeated calls allocate nothing; calls on one handle are serialised. */
+int mf_der_lines_host(const mf_der *dc, int order, int sub, const double *x, double *out,
+                      int nlines, long long ld);
 /* mf_der3d of nfields (1..8) fields of one shape in one launch: x[f] -> out[f]
  * (host arrays of device pointers; no output may alias an input).  The
  * line recurrences are latency-bound, so fields batched together run in

@@ -19,6 +19,7 @@
 
 #include "der_seg.h"
 #include "internal.h"
+#include "stage.h"
 #include "mflops/mf.h"
 
 namespace mf {
@@ -360,6 +361,7 @@ __global__ void __launch_bounds__(256) der_lines_kernel(const double *__restrict
 // ---------------------------------------------------------------------------
 struct mf_der {
   int n = 0, so = 6, solver = 0;
+  mutable mf::HostStage stage;  // device staging of the host-array entry points
   double *dbuf = nullptr;      // device tables, then one scratch line of n
   double *scratch = nullptr;   // surplus lanes' output line
   mf::LineTab tab[2][2];       // [order-1][sub]
@@ -593,25 +595,47 @@ extern "C" int mf_der3d(const mf_der *dc, int axis, int order, const double *x, 
 
 extern "C" int mf_der3d_host(const mf_der *dc, int axis, int order, const double *x, double *out,
                              int nx, int ny, int nz) {
-  if (!x || !out || nx < 1 || ny < 1 || nz < 1) return mf::fail(MF_ERR_ARG, "mf_der3d_host: invalid argument");
+  if (!dc || !x || !out || nx < 1 || ny < 1 || nz < 1)
+    return mf::fail(MF_ERR_ARG, "mf_der3d_host: invalid argument");
   const size_t bytes = (size_t)nx * ny * nz * sizeof(double);
-  double *dx = nullptr, *dy = nullptr;
-  HIPCHK(hipMalloc(&dx, bytes));
-  hipError_t e = hipMalloc(&dy, bytes);
-  if (e != hipSuccess) {
-    (void)hipFree(dx);
-    return mf::fail(MF_ERR_DEVICE, std::string("mf_der3d_host: hipMalloc: ") + hipGetErrorString(e));
-  }
+  std::lock_guard<std::mutex> lock(dc->stage.mu);
+  hipError_t e = dc->stage.reserve(2 * bytes);
+  if (e != hipSuccess)
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_der3d_host: staging: ") + hipGetErrorString(e));
+  double *dx = (double *)dc->stage.dev, *dy = dx + (size_t)nx * ny * nz;
   int rc = MF_OK;
   e = hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     rc = mf_der3d(dc, axis, order, dx, dy, nx, ny, nz, nullptr);
     if (rc == MF_OK) e = hipMemcpy(out, dy, bytes, hipMemcpyDeviceToHost);
   }
-  (void)hipFree(dx);
-  (void)hipFree(dy);
   if (rc != MF_OK) return rc;
   if (e != hipSuccess)
     return mf::fail(MF_ERR_DEVICE, std::string("mf_der3d_host: ") + hipGetErrorString(e));
+  return MF_OK;
+}
+
+extern "C" int mf_der_lines_host(const mf_der *dc, int order, int sub, const double *x, double *out,
+                                 int nlines, long long ld) {
+  if (!dc || !x || !out || nlines < 0 || ld < dc->n)
+    return mf::fail(MF_ERR_ARG, "mf_der_lines_host: invalid argument");
+  if (nlines == 0) return MF_OK;
+  const size_t bytes = (size_t)nlines * (size_t)ld * sizeof(double);
+  std::lock_guard<std::mutex> lock(dc->stage.mu);
+  hipError_t e = dc->stage.reserve(2 * bytes);
+  if (e != hipSuccess)
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_der_lines_host: staging: ") + hipGetErrorString(e));
+  double *dx = (double *)dc->stage.dev, *dy = dx + (size_t)nlines * ld;
+  e = hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice);
+  // the sub-line scheme leaves the end points of out untouched
+  if (e == hipSuccess && sub) e = hipMemcpy(dy, out, bytes, hipMemcpyHostToDevice);
+  int rc = MF_OK;
+  if (e == hipSuccess) {
+    rc = mf_der_lines(dc, order, sub, dx, dy, nlines, ld, nullptr);
+    if (rc == MF_OK) e = hipMemcpy(out, dy, bytes, hipMemcpyDeviceToHost);
+  }
+  if (rc != MF_OK) return rc;
+  if (e != hipSuccess)
+    return mf::fail(MF_ERR_DEVICE, std::string("mf_der_lines_host: ") + hipGetErrorString(e));
   return MF_OK;
 }

@@ -36,6 +36,7 @@
 #include "internal.h"
 #include "residual.h"
 #include "residual_zs.h"
+#include "stage.h"
 #include "mflops/mf.h"
 
 namespace mf {
@@ -191,6 +192,7 @@ __global__ void __launch_bounds__(1024) sum_kernel(const double *__restrict__ pa
 // ---------------------------------------------------------------------------
 struct mf_solver3d {
   int nx = 0, ny = 0, nz = 0, n1 = 0, n2 = 0, n3 = 0, so = 6;
+  mf::HostStage stage;  // device staging of mf_solver3d_solve_host
   int bct[6] = {1, 1, 1, 1, 1, 1};
   mf::Solver1D ax[3];
   // device memory
@@ -1216,8 +1218,9 @@ extern "C" int mf_solver3d_solve_host(mf_solver3d *h, const double *g, double si
   const size_t bx = (size_t)2 * h->n2 * h->n3 * sizeof(double);
   const size_t by = (size_t)2 * h->n1 * h->n3 * sizeof(double);
   const size_t bz = (size_t)2 * h->n1 * h->n2 * sizeof(double);
-  char *d = nullptr;
-  HIPCHK(hipMalloc(&d, 2 * full + bx + by + bz));
+  std::lock_guard<std::mutex> lock(h->stage.mu);
+  HIPCHK(h->stage.reserve(2 * full + bx + by + bz));
+  char *d = (char *)h->stage.dev;
   double *dg = (double *)d, *du = (double *)(d + full);
   double *dbx = (double *)(d + 2 * full), *dby = (double *)(d + 2 * full + bx);
   double *dbz = (double *)(d + 2 * full + bx + by);
@@ -1234,7 +1237,6 @@ extern "C" int mf_solver3d_solve_host(mf_solver3d *h, const double *g, double si
       if (e2 != hipSuccess) e = e2;
     }
   }
-  (void)hipFree(d);
   if (e != hipSuccess)
     return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_solve_host: ") + hipGetErrorString(e));
   return rc;

@@ -60,6 +60,16 @@ module mflops_c
        real(c_double), intent(inout) :: out(*)
        integer(c_int) :: rc
      end function mf_der3d_host
+     function mf_der_lines_host(h, order, sub, x, out, nlines, ld) bind(c, name="mf_der_lines_host") &
+          result(rc)
+       import :: c_double, c_int, c_ptr, c_long_long
+       type(c_ptr), value :: h
+       integer(c_int), value :: order, sub, nlines
+       integer(c_long_long), value :: ld
+       real(c_double), intent(in) :: x(*)
+       real(c_double), intent(inout) :: out(*)
+       integer(c_int) :: rc
+     end function mf_der_lines_host
      function mf_solver3d_create(gx, nx, gy, ny, gz, nz, bct, so, h) &
           bind(c, name="mf_solver3d_create") result(rc)
        import :: c_double, c_int, c_ptr
@@ -150,36 +160,24 @@ contains
     dc%h = c_null_ptr
   end subroutine der_coeffs_destroy
 
-  ! One line through the 3-D x-driver: in a (n,3,3) stack the middle line
-  ! (2,2) is interior and gets the full scheme; line (1,2) lies in one
-  ! boundary plane and gets the sub-line scheme of a solver-mode handle.
+  ! One line (der/dder and the sub-line der_s/dder_s, which leaves dx(1) and
+  ! dx(n) untouched) through mf_der_lines_host: the handle's staging buffer
+  ! is reused, so a loop over the lines of a field allocates nothing.
   subroutine line(dc, x, dx, order, sub)
     type(derivatives_coefficients), intent(in) :: dc
     real(c_double), intent(in) :: x(:)
     real(c_double), intent(inout) :: dx(:)
     integer, intent(in) :: order
     logical, intent(in) :: sub
-    real(c_double), allocatable :: f(:,:,:), d(:,:,:)
-    integer :: n
-    n = dc%n
+    integer(c_int) :: s
     if (sub .and. .not. dc%solver) then
        print '(a)', 'der_s/dder_s need coefficients built with solve=.true.'
        stop 1
     end if
-    allocate(f(n,3,3), d(n,3,3))
-    f = 0.0_c_double
-    if (sub) then
-       f(:,1,2) = x(1:n)
-    else
-       f(:,2,2) = x(1:n)
-    end if
-    call mf_check(mf_der3d_host(dc%h, 0_c_int, int(order, c_int), f, d, int(n, c_int), 3_c_int, &
-         3_c_int), 'der')
-    if (sub) then
-       dx(2:n-1) = d(2:n-1,1,2)
-    else
-       dx(1:n) = d(:,2,2)
-    end if
+    s = 0
+    if (sub) s = 1
+    call mf_check(mf_der_lines_host(dc%h, int(order, c_int), s, x, dx, 1_c_int, &
+         int(dc%n, c_long_long)), 'der')
   end subroutine line
 
   subroutine der(dc, x, dx)

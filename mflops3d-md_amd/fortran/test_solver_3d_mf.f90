@@ -104,6 +104,47 @@ program test_solver_3d_mf
         e = sqrt(sum((df(2:nx-1,2:ny-1,2:nz-1) - dex(2:nx-1,2:ny-1,2:nz-1))**2) / &
              sum(dex(2:nx-1,2:ny-1,2:nz-1)**2))
         print '(a,i1,a,i1,a,es24.16)', 'derivative_axis', axis, '_order', order, ' ', e
+        if (axis == 0) then
+           ! the reference's line loop (class_field.f90:649-684): der on every
+           ! interior x-line, der_s on the lines lying in one boundary plane
+           block
+             real(c_double), allocatable :: dl(:,:,:)
+             integer(8) :: c0, c1, rate
+             integer :: nl
+             allocate(dl(nx,ny,nz))
+             dl = 0.0_c_double
+             nl = 0
+             call system_clock(c0, rate)
+             do k = 1, nz
+                do j = 1, ny
+                   if ((j == 1 .or. j == ny) .and. (k == 1 .or. k == nz)) cycle
+                   if (j == 1 .or. j == ny .or. k == 1 .or. k == nz) then
+                      if (order == 1) call der_s(dc, f(:,j,k), dl(:,j,k))
+                      if (order == 2) call dder_s(dc, f(:,j,k), dl(:,j,k))
+                   else
+                      if (order == 1) call der(dc, f(:,j,k), dl(:,j,k))
+                      if (order == 2) call dder(dc, f(:,j,k), dl(:,j,k))
+                   end if
+                   nl = nl + 1
+                end do
+             end do
+             call system_clock(c1)
+             e = 0.0_c_double
+             do k = 1, nz
+                do j = 1, ny
+                   if ((j == 1 .or. j == ny) .and. (k == 1 .or. k == nz)) cycle
+                   if (j == 1 .or. j == ny .or. k == 1 .or. k == nz) then
+                      e = max(e, maxval(abs(dl(2:nx-1,j,k) - df(2:nx-1,j,k))))
+                   else
+                      e = max(e, maxval(abs(dl(:,j,k) - df(:,j,k))))
+                   end if
+                end do
+             end do
+             print '(a,i1,a,es24.16)', 'der_lines_order', order, '_vs_3d ', e / maxval(abs(df))
+             print '(a,i1,a,es24.16)', 'der_lines_order', order, '_us_per_line ', &
+                  1.0d6 * real(c1 - c0, 8) / real(rate, 8) / real(nl, 8)
+           end block
+        end if
         call der_coeffs_destroy(dc)
      end do
   end do

@@ -73,6 +73,7 @@ SIGNATURES = {
     "mf_ns_set_les": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
     "mf_ns_set_scheme": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "mf_ns_set_sgs_coupling": (C.c_int, [_vp, C.c_int]),
+    "mf_der_lines_host": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_longlong]),
     "mf_sgs_eddy_targets": (C.c_int, [_vp, C.c_longlong, C.c_double, _vp, _vp]),
     "mf_ns_subiterations": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mf_ns_restart_write": (C.c_int, [_vp, C.c_char_p, _vp, C.c_int, _vp]),
