@@ -18,7 +18,10 @@ from the first five transforms of a solve launched back to back between two
 HIP events; per-launch event pairs, used only for the breakdown, add the
 host's enqueue gaps)
 and the CPU baseline (the FP64 oracle on the host's cores, a bounded sample of
-the same workload).
+the same workload).  Extra keys: poisson_sigma0 (the same problem at sigma = 0,
+FP64 and the fastest mixed variant), influence (the 4x4x2-subdomain
+influence-matrix assembly, responses/s of device time) and ns_steps_per_s
+(100 timed steps of the 192x256x192 channel, FP64 and fp16x2 velocity solves).
 """
 from __future__ import annotations
 
@@ -74,7 +77,10 @@ def main():
     ap.add_argument("--sgs-steps", type=int, default=200)
     ap.add_argument("--sgs-snap", type=int, default=50)
     ap.add_argument("--sgs-train", type=int, default=300)
-    ap.add_argument("--ns-steps", type=int, default=10, help="timed NS steps (0: skip)")
+    ap.add_argument("--ns-steps", type=int, default=100, help="timed NS steps (0: skip)")
+    ap.add_argument("--no-poisson", action="store_true", help="skip the sigma=0 solves")
+    ap.add_argument("--no-influence", action="store_true",
+                    help="skip the influence-matrix assembly key")
     ap.add_argument("--ns-warmup", type=int, default=3)
     ap.add_argument("--ns-shape", type=int, nargs=3, default=(192, 256, 192))
     ap.add_argument("--copy-probe", action="store_true",
@@ -252,11 +258,50 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"], errs = cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx[best])
         out.update(errs)
+    if not args.no_poisson:
+        out["poisson_sigma0"] = poisson_segment(args, sc, gd, bc, best, dict(variants)[best], inner)
+    del sc, gd, u64, umx
+    torch.cuda.empty_cache()
+    if not args.no_influence:
+        out["influence"] = influence_key(args)
     if args.ns_steps > 0:
-        del sc, gd, u64, umx
-        torch.cuda.empty_cache()
         out["ns_steps_per_s"] = ns_segment(args)
     print(json.dumps(out), flush=True)
+
+
+def poisson_segment(args, sc, gd, bc, name, prec, inner):
+    """The same 256^3 problem at sigma = 0 (the Poisson operator, the worst
+    conditioned case of the solver): FP64 and the fastest mixed variant, each
+    args.steps timed solves after one untimed one; inner solves, residual and
+    the mixed result's distance to the FP64 one."""
+    import numpy as np
+    import torch
+
+    import mflops as mf
+
+    n = gd.shape[0]
+    res = {"sigma": 0.0, "inner_solve": name}
+    outs = {}
+    for key, p in (("fp64", mf.MF_FP64), (name, prec)):
+        u = mf.fortran_zeros((n, n, n))
+        sc.solve(gd, u, bc, 0.0, p, tol=args.tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rep = sc.solve(gd, u, bc, 0.0, p, tol=args.tol)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        res[key] = {"value": round(1.0 / t, 3), "ms_per_step": round(t * 1e3, 4)}
+        if p != mf.MF_FP64:
+            res[key].update(inner_solves=rep.iterations, residual=rep.residual,
+                            residual_after_first=rep.residual0)
+        outs[key] = mf.to_host(u)
+        del u
+        log("poisson sigma=0 %s: %.3f ms/solve" % (key, t * 1e3))
+    a, b = outs["fp64"][inner], outs[name][inner]
+    res["mixed_over_fp64"] = round(res[name]["value"] / res["fp64"]["value"], 3)
+    res["err_vs_fp64_gpu"] = float(np.abs(a - b).max() / np.abs(a).max())
+    return res
 
 
 def ns_segment(args, cpu_steps=0):
@@ -475,6 +520,60 @@ def run_md(args):
     print(json.dumps(out), flush=True)
 
 
+def influence_assembly(reps):
+    """The influence matrix of the 4x4x2 = 32-subdomain layout (per-slab
+    sizes drawn from {48,64,96}, seed 1, asin-stretched FP64 grids, Dirichlet
+    interfaces, sigma of the NS velocity solve), formed reps times; returns
+    the layout, the response count and the fastest run's device time, flops,
+    block bytes and wall time of the whole create."""
+    import torch
+
+    import mflops as mf
+    from mflops import synthetic
+
+    sigma = -3000.0 * 1.5 / 1e-3
+    slab, subs = synthetic.subdomain_layout()
+    nd = tuple(len(v) for v in slab)
+    grids = [[None] * nd[ax] for ax in range(3)]
+    for sd in subs:
+        for ax in range(3):
+            grids[ax][sd["index"][ax]] = sd["grids"][ax]
+    log("influence: slab sizes x=%s y=%s z=%s" % tuple(slab))
+    responses = 0
+    for sd in subs:
+        shape = tuple(len(g) for g in sd["grids"])
+        responses += sum(mf.face_points(shape, f) for f in sd["interfaces"])
+    times, walls = [], []
+    h = None
+    for _ in range(reps):
+        del h
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h = mf.Multidomain(grids, sigma)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        ms, flop, nbytes = h.assembly_stats()
+        times.append(ms)
+    del h
+    torch.cuda.empty_cache()
+    ms = min(times)
+    log("influence: %d impulse responses, %.1f GB of blocks; device assembly %.1f ms (runs %s), "
+        "create incl. host set-up %.2f s" % (responses, nbytes / 1e9, ms, ["%.1f" % t for t in times],
+                                           min(walls)))
+    return dict(sigma=sigma, slab=slab, subs=subs, responses=responses, ms=ms, flop=flop,
+                nbytes=nbytes, wall=min(walls))
+
+
+def influence_key(args):
+    """The influence assembly's figures for the headline line (two runs)."""
+    a = influence_assembly(2)
+    tflops = a["flop"] / (a["ms"] * 1e-3) / 1e12
+    return {"workload": "influence-4x4x2-{48,64,96}-seed1", "value": round(a["responses"] / (a["ms"] * 1e-3), 1),
+            "unit": "responses/s", "responses": a["responses"], "device_ms": round(a["ms"], 2),
+            "fp64_tflops": round(tflops, 2), "frac_fp64_peak": round(tflops / PEAK["fp64_mfma_tflops"], 4),
+            "create_s_incl_host_setup": round(a["wall"], 3)}
+
+
 def run_influence(args):
     """Influence-matrix decomposition on one GPU: 4x4x2 = 32 subdomains with
     per-slab sizes drawn from {48,64,96} (seed 1), asin-stretched FP64 grids,
@@ -491,38 +590,12 @@ def run_influence(args):
     import torch
 
     import mflops as mf
-    from mflops import synthetic
 
     torch.cuda.set_device(0)
-    sigma = -3000.0 * 1.5 / 1e-3
-    slab, subs = synthetic.subdomain_layout()
-    nd = tuple(len(v) for v in slab)
-    grids = [[None] * nd[ax] for ax in range(3)]
-    for sd in subs:
-        for ax in range(3):
-            grids[ax][sd["index"][ax]] = sd["grids"][ax]
-    log("influence: slab sizes x=%s y=%s z=%s" % tuple(slab))
-    responses = 0
-    for sd in subs:
-        shape = tuple(len(g) for g in sd["grids"])
-        u = sum(mf.face_points(shape, f) for f in sd["interfaces"])
-        responses += u
-    reps = max(1, min(args.steps, 3))
-    times, walls = [], []
-    h = None
-    for _ in range(reps):
-        del h
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        h = mf.Multidomain(grids, sigma)
-        torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
-        ms, flop, nbytes = h.assembly_stats()
-        times.append(ms)
-    ms = min(times)
-    log("influence: %d impulse responses, %.1f GB of blocks; device assembly %.1f ms (runs %s), "
-        "create incl. host set-up %.2f s" % (responses, nbytes / 1e9, ms, ["%.1f" % t for t in times],
-                                           min(walls)))
+    a = influence_assembly(max(1, min(args.steps, 3)))
+    sigma, slab, subs, responses = a["sigma"], a["slab"], a["subs"], a["responses"]
+    ms, flop, nbytes = a["ms"], a["flop"], a["nbytes"]
+    walls = [a["wall"]]
     tflops = flop / (ms * 1e-3) / 1e12
     # the per-impulse solve path on a bounded sample: the largest subdomain
     big = max(subs, key=lambda sd: np.prod([len(g) for g in sd["grids"]]))
