@@ -29,6 +29,7 @@
 #include "gemm_bf3.h"
 #include "gemm_h2.h"
 #include "gemm_h2r.h"
+#include "gemm_h2p.h"
 #include "gemm_h2z.h"
 #include "infl_spec.h"
 #include "infl_tune.h"
@@ -209,6 +210,9 @@ struct mf_solver3d {
   // same layout
   _Float16 *Whs[3] = {nullptr, nullptr, nullptr}, *Vhs[3] = {nullptr, nullptr, nullptr};
   float wsc[3][2] = {{1, 1}, {1, 1}, {1, 1}};
+  // the forward y and backward x fp16 splits again with k permuted within
+  // each 32-wide k-step (h2p_perm), the second operands of the plane pairs
+  _Float16 *Whp1 = nullptr, *Vhp0 = nullptr;
   // largest magnitudes (float bits) of the fp16 path's fields: [0] r' / b',
   // [1..5] the outputs of the first five transforms
   unsigned *fmx = nullptr;
@@ -234,6 +238,7 @@ struct mf_solver3d {
   // null rows w0x | w0y | w0z of inv(V), 1 / (sum w0x sum w0y sum w0z), and
   // n3 plane partials
   bool h2_resident = true;  // fp16x2 transforms with the eigenvectors in registers (MF_H2_RESIDENT=0: off)
+  bool h2_plane = true;     // fp16x2 x/y transform pairs per z-plane (MF_H2_PLANE=0: off)
   int ncu = 256;        // compute units of the device the handle was created on
   bool null_mode = false;
   double *w0 = nullptr, *npart = nullptr;
@@ -528,6 +533,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
       }
   }
   if (const char *env = std::getenv("MF_H2_RESIDENT")) h->h2_resident = std::atoi(env) != 0;
+  if (const char *env = std::getenv("MF_H2_PLANE")) h->h2_plane = std::atoi(env) != 0;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -599,6 +605,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
       h->Whs[a] = (_Float16 *)take(cnh);
       h->Vhs[a] = (_Float16 *)take(cnh);
     }
+    h->Whp1 = (_Float16 *)take((2 * (size_t)h->wrows[1] * h->kp[1] + 3) / 4);
+    h->Vhp0 = (_Float16 *)take((2 * (size_t)h->wrows[0] * h->kp[0] + 3) / 4);
     h->fmx = (unsigned *)take(4);
     h->w0 = take((size_t)n[0] + n[1] + n[2]);
     h->npart = take(n[2]);
@@ -667,6 +675,18 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
         }
       if (e == hipSuccess)
         e = up(which == 0 ? (void *)h->Whs[a] : (void *)h->Vhs[a], hp.data(), hp.size() * 2);
+      if (e == hipSuccess && ((a == 1 && which == 0) || (a == 0 && which == 1))) {
+        // k permuted within each 32-wide k-step: position kk holds k = h2p_perm(kk)
+        std::vector<uint16_t> pp(hp.size(), 0);
+        for (int part = 0; part < 2; ++part)
+          for (int kk = 0; kk < kp; ++kk) {
+            const int k = 32 * (kk / 32) + mf::h2p_perm(kk % 32);
+            for (int r = 0; r < wr; ++r)
+              pp[(((size_t)part * (kp / 8) + kk / 8) * wr + r) * 8 + kk % 8] =
+                  hp[(((size_t)part * (kp / 8) + k / 8) * wr + r) * 8 + k % 8];
+          }
+        e = up(a == 1 ? (void *)h->Whp1 : (void *)h->Vhp0, pp.data(), pp.size() * 2);
+      }
     }
   }
   // null rows of a singular problem (the zero eigenvalues are exact zeros)
@@ -915,6 +935,72 @@ void launch_h2(mf_solver3d *h, const TF *F, long ldf, long sF, const _Float16 *W
 // one kernel (gemm_h2z_kernel) when one block holds every z of both products.
 bool h2_zfused(const mf_solver3d *h) { return h->kp[2] <= 256 && h->wrows[2] == mf::H2Z_BN; }
 
+// The x and y transforms of each direction run as one kernel per z-plane
+// (gemm_h2p.h) when both eigenvector dimensions take 256-row splits (129..256
+// points, k padded to 192 or 256) and there are planes for at least half
+// the CUs (one block per plane: at 38 planes the pair was 18% slower than
+// the separate transforms, at 98 equal, at 148 4% faster).
+bool h2_planes(const mf_solver3d *h) {
+  return h->h2_plane && h->wrows[0] == mf::H2P_N && h->wrows[1] == mf::H2P_N &&
+         (h->kp[0] == 192 || h->kp[0] == 256) && (h->kp[1] == 192 || h->kp[1] == 256) &&
+         2 * h->n3 >= h->ncu;
+}
+
+// forward x+y pair: in (slot 0) -> out (slot 2)
+template <int KPX, int KPY>
+void launch_h2p_fwd_k(mf_solver3d *h, const float *in, float *out, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  ProfScope ps(h, P_T32, 2.0 * n1 * n2 * n3 * (n1 + n2), st);
+  hipLaunchKernelGGL((mf::h2p_fwd_kernel<8, KPX, KPY>), dim3((unsigned)std::min(n3, h->ncu)),
+                     dim3(512), 0, st, in, n1, n2, n3, (const _Float16 *)h->Whs[0],
+                     (const _Float16 *)h->Whp1, h->wsc[0][0], h->wsc[1][0], out,
+                     (const unsigned *)(h->fmx + 0), h->fmx + 2);
+}
+void launch_h2p_fwd(mf_solver3d *h, const float *in, float *out, hipStream_t st) {
+  if (h->kp[0] == 256) {
+    if (h->kp[1] == 256) launch_h2p_fwd_k<256, 256>(h, in, out, st);
+    else launch_h2p_fwd_k<256, 192>(h, in, out, st);
+  } else {
+    if (h->kp[1] == 256) launch_h2p_fwd_k<192, 256>(h, in, out, st);
+    else launch_h2p_fwd_k<192, 192>(h, in, out, st);
+  }
+}
+
+// backward y+x pair: in (the largest magnitude in slot 4) -> u (written or
+// added); clears slot 0
+template <int KPX, int KPY, bool ADD>
+void launch_h2p_bwd_k(mf_solver3d *h, const float *in, double *u, hipStream_t st) {
+  const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
+  ProfScope ps(h, P_T32, 2.0 * n1 * n2 * n3 * (n1 + n2), st);
+  hipLaunchKernelGGL((mf::h2p_bwd_kernel<8, KPX, KPY, ADD>), dim3((unsigned)std::min(n3, h->ncu)),
+                     dim3(512), 0, st, in, n1, n2, n3, (const _Float16 *)h->Vhs[1],
+                     (const _Float16 *)h->Vhp0, h->wsc[1][1], h->wsc[0][1], u, h->nx, h->ny,
+                     (const unsigned *)(h->fmx + 4), h->fmx);
+}
+template <bool ADD>
+void launch_h2p_bwd_a(mf_solver3d *h, const float *in, double *u, hipStream_t st) {
+  if (h->kp[0] == 256) {
+    if (h->kp[1] == 256) launch_h2p_bwd_k<256, 256, ADD>(h, in, u, st);
+    else launch_h2p_bwd_k<256, 192, ADD>(h, in, u, st);
+  } else {
+    if (h->kp[1] == 256) launch_h2p_bwd_k<192, 256, ADD>(h, in, u, st);
+    else launch_h2p_bwd_k<192, 192, ADD>(h, in, u, st);
+  }
+}
+void launch_h2p_bwd(mf_solver3d *h, const float *in, double *u, bool accumulate, hipStream_t st) {
+  if (accumulate) launch_h2p_bwd_a<true>(h, in, u, st);
+  else launch_h2p_bwd_a<false>(h, in, u, st);
+}
+
+void launch_h2z(mf_solver3d *h, const float *in, float *out, const EpiArgs &ep, hipStream_t st) {
+  const long n12 = (long)h->n1 * h->n2;
+  ProfScope ps(h, P_T32, 4.0 * n12 * h->n3 * h->n3, st);
+  hipLaunchKernelGGL(mf::gemm_h2z_kernel, dim3((unsigned)((n12 + mf::H2Z_BM - 1) / mf::H2Z_BM)),
+                     dim3(256), 0, st, in, n12, (const _Float16 *)h->Whs[2],
+                     (const _Float16 *)h->Vhs[2], h->kp[2], h->wrows[2], out, n12, (int)n12, h->n3,
+                     ep, (const unsigned *)(h->fmx + 2), h->wsc[2][0], h->wsc[2][1], h->fmx + 4);
+}
+
 // The six transforms of transforms_bf3 on the two-part fp16 matrices.  Field
 // scales: slot 0 holds max |r'| (written by the residual kernel), slot q the
 // largest output of transform q (the fused z pair writes slot 4); the last
@@ -938,16 +1024,24 @@ void transforms_h2(mf_solver3d *h, const float *in, float *w1, float *w2, double
   ep.nx = h->nx;
   ep.ny = h->ny;
   using mf::EPI_STORE;
+  if (h2_planes(h) && u) {
+    // x+y forward pair, the z transforms (one kernel when fused), y+x
+    // backward pair into u
+    launch_h2p_fwd(h, in, w2, st);
+    if (h2_zfused(h)) {
+      launch_h2z(h, w2, w1, ep, st);
+      launch_h2p_bwd(h, w1, u, accumulate, st);
+    } else {
+      launch_h2<float, float, false, mf::EPI_DIVIDE>(h, w2, n12, 0, h->Whs[2], h->wsc[2][0], kz, rz, w1, n12, 0, (int)n12, n3, n3, 1, ep, 2, 3, false, st);
+      launch_h2<float, float, false, EPI_STORE>(h, w1, n12, 0, h->Vhs[2], h->wsc[2][1], kz, rz, w2, n12, 0, (int)n12, n3, n3, 1, ep, 3, 4, false, st);
+      launch_h2p_bwd(h, w2, u, accumulate, st);
+    }
+    return;
+  }
   launch_h2<float, float, true, EPI_STORE>(h, in, n1, 0, h->Whs[0], h->wsc[0][0], kx, rx, w1, n1, 0, n1, n2 * n3, n1, 1, ep, 0, 1, false, st);
   launch_h2<float, float, false, EPI_STORE>(h, w1, n1, n12, h->Whs[1], h->wsc[1][0], ky, ry, w2, n1, n12, n1, n2, n2, n3, ep, 1, 2, false, st);
   if (h2_zfused(h)) {
-    {
-      ProfScope ps(h, P_T32, 4.0 * n12 * n3 * n3, st);
-      hipLaunchKernelGGL(mf::gemm_h2z_kernel, dim3((unsigned)((n12 + mf::H2Z_BM - 1) / mf::H2Z_BM)),
-                         dim3(256), 0, st, (const float *)w2, n12, (const _Float16 *)h->Whs[2],
-                         (const _Float16 *)h->Vhs[2], kz, rz, w1, n12, (int)n12, n3, ep,
-                         (const unsigned *)(h->fmx + 2), h->wsc[2][0], h->wsc[2][1], h->fmx + 4);
-    }
+    launch_h2z(h, w2, w1, ep, st);
     launch_h2<float, float, false, EPI_STORE>(h, w1, n1, n12, h->Vhs[1], h->wsc[1][1], ky, ry, w2, n1, n12, n1, n2, n2, n3, ep, 4, 5, false, st);
     if (!last) return;
     if (accumulate)

@@ -287,6 +287,43 @@ def test_solve_mixed_residual_kernels(monkeypatch, cz, shape):
         assert rep.residual <= 1e-13 and true_res <= 2e-13
 
 
+@pytest.mark.parametrize("shape", [(140, 200, 136), (258, 150, 130)])
+def test_solve_fp16x2_plane_pairs(monkeypatch, shape):
+    """The fp16x2 x/y transform pairs per z-plane (x and y of 129..256
+    points, k padded to 192 and to 256; 134 planes with the fused z pair,
+    128 planes with separate z transforms) against the separate transforms
+    (MF_H2_PLANE=0) and the FP64 solve: the residual contract checked by the
+    80-bit residual, 2e-11 against FP64, and the two paths differing (so
+    that both ran)."""
+    mf, torch = _mf()
+    bct, sigma = (1, 2, 1, 1, 2, 1), -300.0
+    nx, ny, nz = shape
+    grids = (o.grid_uniform(nx, 0, 1), o.grid_tanh(ny, 0, 1, 1.5), o.grid_asin(nz, 0, 0.5))
+    sc = o.SolverCoeffs3D(*grids, bct, 6)
+    rng = np.random.default_rng(14)
+    g = np.asfortranarray(rng.standard_normal(shape))
+    bcs = tuple(np.asfortranarray(rng.standard_normal(s))
+                for s in ((ny - 2, nz - 2, 2), (nx - 2, nz - 2, 2), (nx - 2, ny - 2, 2)))
+    print("run fp64 shape=%s" % (shape,), flush=True)
+    u64, _ = _solve_gpu(mf, grids, bct, g, bcs, sigma, mf.MF_FP64)
+    inner = (slice(1, -1),) * 3
+    scale = np.abs(u64[inner]).max()
+    got = {}
+    for plane in ("1", "0"):
+        monkeypatch.setenv("MF_H2_PLANE", plane)
+        print("run fp16x2 plane pairs=%s shape=%s" % (plane, shape), flush=True)
+        u, rep = _solve_gpu(mf, grids, bct, g, bcs, sigma, MIXED["fp16x2"])
+        res = _true_residual(sc, u, g, bcs, sigma)
+        err = float(np.abs(u[inner] - u64[inner]).max() / scale)
+        print("  %d inner solves, residual %.2e, 80-bit residual %.2e, vs fp64 %.2e"
+              % (rep.iterations, rep.residual, res, err), flush=True)
+        assert rep.residual <= 1e-12 and res <= 2e-12 and err <= 2e-11
+        got[plane] = u
+    diff = float(np.abs(got["1"][inner] - got["0"][inner]).max() / scale)
+    print("  plane pairs vs separate transforms: %.2e" % diff, flush=True)
+    assert 0.0 < diff <= 2e-11
+
+
 def test_solve_c1_helmholtz_and_poisson():
     """Config 1: sigma=1e3 Helmholtz and sigma=0 Poisson, RHS = operator(u)."""
     mf, torch = _mf()
