@@ -951,7 +951,7 @@ template <int KPX, int KPY>
 void launch_h2p_fwd_k(mf_solver3d *h, const float *in, float *out, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   ProfScope ps(h, P_T32, 2.0 * n1 * n2 * n3 * (n1 + n2), st);
-  hipLaunchKernelGGL((mf::h2p_fwd_kernel<8, KPX, KPY>), dim3((unsigned)std::min(n3, h->ncu)),
+  hipLaunchKernelGGL((mf::h2p_fwd_kernel<KPX, KPY>), dim3((unsigned)n3),
                      dim3(512), 0, st, in, n1, n2, n3, (const _Float16 *)h->Whs[0],
                      (const _Float16 *)h->Whp1, h->wsc[0][0], h->wsc[1][0], out,
                      (const unsigned *)(h->fmx + 0), h->fmx + 2);
@@ -972,7 +972,7 @@ template <int KPX, int KPY, bool ADD>
 void launch_h2p_bwd_k(mf_solver3d *h, const float *in, double *u, hipStream_t st) {
   const int n1 = h->n1, n2 = h->n2, n3 = h->n3;
   ProfScope ps(h, P_T32, 2.0 * n1 * n2 * n3 * (n1 + n2), st);
-  hipLaunchKernelGGL((mf::h2p_bwd_kernel<8, KPX, KPY, ADD>), dim3((unsigned)std::min(n3, h->ncu)),
+  hipLaunchKernelGGL((mf::h2p_bwd_kernel<KPX, KPY, ADD>), dim3((unsigned)n3),
                      dim3(512), 0, st, in, n1, n2, n3, (const _Float16 *)h->Vhs[1],
                      (const _Float16 *)h->Vhp0, h->wsc[1][1], h->wsc[0][1], u, h->nx, h->ny,
                      (const unsigned *)(h->fmx + 4), h->fmx);
