@@ -13,11 +13,12 @@ a relative residual <= 1e-12; the FP64 solve is the reference algorithm in
 FP64 on MFMA.
 
 Prints one JSON line (value = the faster mixed variant's solves/s) with the
-roofline of its dominant kernel (the transform GEMM; its average launch time
-from the first five transforms of a solve launched back to back between two
-HIP events; per-launch event pairs, used only for the breakdown, add the
-host's enqueue gaps)
-and the CPU baseline (the FP64 oracle on the host's cores, a bounded sample of
+roofline of its dominant kernel (the FP64 residual, two launches per solve
+and the largest single-kernel share of the solve; its average launch time
+from HIP event pairs around each launch in a second pass of the same
+solves), the roofline of its transform kernels (transform_roofline: the
+launches of an inner solve's transforms back to back between two HIP
+events) and the CPU baseline (the FP64 oracle on the host's cores, a bounded sample of
 the same workload).  Extra keys: poisson_sigma0 (the same problem at sigma = 0,
 FP64 and the fastest mixed variant), influence (the 4x4x2-subdomain
 influence-matrix assembly, responses/s of device time) and ns_steps_per_s
@@ -178,10 +179,17 @@ def main():
     err_mx_64 = {name: float(np.abs(a[inner] - a64[inner]).max() / scale) for name, a in amx.items()}
 
     # the transform kernels' own durations: the first five transforms of a
-    # solve launched back to back (per-launch event pairs would add the
-    # host's enqueue gaps)
+    # solve launched back to back, or for the fp16x2 plane pairs an inner
+    # solve's three launches (per-launch event pairs would add the host's
+    # enqueue gaps)
     kt = {name: sc.time_transforms(prec, reps=10) for name, prec in
           (("fp64", mf.MF_FP64),) + variants}
+    npts = (n - 2) ** 3
+    # the plane pairs ran if the fp16x2 timing covered six transforms in
+    # three launches: x+y forward (8 B/point), z pair (8), y+x backward into
+    # FP64 u (12)
+    f_planes = 2.0 * npts * 6 * (n - 2) / 3
+    planes = abs(kt["fp16x2"][1] - f_planes) < 1e-6 * f_planes
 
     def roof(name, peak, ops=1.0, traf=None, bytes_per_point=8.0):
         """ops: hardware MFMA flops per algorithmic flop (6 for bf16x3, 3 for
@@ -190,7 +198,6 @@ def main():
         read and one write of the field (bytes_per_point in all)."""
         ms, flops = kt[name]
         avg = ms * 1e-3
-        npts = (n - 2) ** 3
         t_mfma = ops * flops / (peak * 1e12)
         t_hbm = bytes_per_point * npts / (PEAK["hbm_gbs"] * 1e9)
         common = {"traffic": traf, "avg_launch_ms": round(ms, 4), "flops_per_launch": flops,
@@ -204,16 +211,33 @@ def main():
         return dict({"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK["hbm_gbs"],
                      "unit": "GB/s", "frac": round(ach / PEAK["hbm_gbs"], 4)}, **common)
 
+    if planes:
+        tr = [measured_traffic(k) for k in ("h2p_fwd_kernel", "gemm_h2z_kernel", "h2p_bwd_kernel<store>")]
+        traf_h2 = sum(tr) / 3 if all(t is not None for t in tr) else None
+        h2 = roof("fp16x2", PEAK["fp16_mfma_tflops"], 3.0, traf=traf_h2, bytes_per_point=28.0 / 3)
+        h2["kernels"] = "h2p_fwd_kernel, gemm_h2z_kernel, h2p_bwd_kernel (store), one launch each"
+    else:
+        h2 = roof("fp16x2", PEAK["fp16_mfma_tflops"], 3.0, traf=measured_traffic("gemm_h2_kernel"))
     roofs = {"fp32": roof("fp32", PEAK["fp32_mfma_tflops"],
                           traf=measured_traffic("gemm_mode_kernel<float")),
              "bf16x3": roof("bf16x3", PEAK["bf16_mfma_tflops"], 6.0,
                             traf=measured_traffic("gemm_bf3_kernel")),
-             "fp16x2": roof("fp16x2", PEAK["fp16_mfma_tflops"], 3.0,
-                            traf=measured_traffic("gemm_h2_kernel"))}
+             "fp16x2": h2}
     r64 = roof("fp64", PEAK["fp64_mfma_tflops"], traf=measured_traffic("gemm_mode_kernel<double"),
                bytes_per_point=16.0)
+    # the fastest mixed solve's dominant kernel: its FP64 residual (two
+    # launches per solve, the largest single-kernel share of the solve time),
+    # timed by the event pairs of the profiled pass
     ms_r, cnt_r, work_r = profmx["residual_fp64"]
     res_gbs = (work_r / cnt_r) / (ms_r / cnt_r * 1e-3) / 1e9 if cnt_r else None
+    res_roof = None
+    if cnt_r:
+        res_roof = {"kernel": "residual_zs_kernel<true>", "bound": "hbm",
+                    "achieved": round(res_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                    "frac": round(res_gbs / PEAK["hbm_gbs"], 4),
+                    "traffic": measured_traffic("residual_zs_kernel<true>"),
+                    "avg_launch_ms": round(ms_r / cnt_r, 4), "bytes_per_launch": work_r / cnt_r,
+                    "launches_per_solve": cnt_r / args.steps}
 
     def breakdown(p):
         return {k: {"ms_per_solve": round(v[0] / args.steps, 4), "launches_per_solve": v[1] / args.steps}
@@ -239,7 +263,8 @@ def main():
                    "interior": n - 2, "sigma": sigma, "tol": args.tol, "so": 6,
                    "inner_solve": best,
                    "rhs": "8 seeded cos modes k=1..16 + 1e-3 noise (seed 2)"},
-        "roofline": roofs[best],
+        "roofline": res_roof if res_roof else roofs[best],
+        "transform_roofline": roofs[best],
         "fp64": {"value": round(1.0 / t64, 3), "unit": "solves/s", "ms_per_step": round(t64 * 1e3, 4),
                  "roofline": r64},
         "mixed_over_fp64": round(t64 / tmx, 3),

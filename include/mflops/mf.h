@@ -274,9 +274,14 @@ int mf_solver3d_profile_read(mf_solver3d *sc, double ms[MF_PROF_CLASSES],
  * handle's own work arrays (their contents are overwritten; no caller data
  * is touched), between two events.  They are five launches, or four for
  * MF_MIXED_FP16X2 when its z pair runs as one kernel (n3 in 129..256); each
- * launch reads one FP32 field and writes one.  ms_per_launch = elapsed /
- * (launches reps); flops_per_launch (may be NULL) = the algorithmic 2 M N K
- * of the five transforms over the launches.  Per-launch event pairs
+ * launch reads one FP32 field and writes one.  When MF_MIXED_FP16X2 runs its
+ * x and y transforms as pairs per z-plane (n1, n2 in 129..256, n3 >= half
+ * the CUs), the timed launches are those of a whole inner solve instead:
+ * the x+y forward pair, the z transforms (one launch when fused) and the
+ * y+x backward pair, which writes an FP64 scratch field of the full size
+ * (allocated for the call).  ms_per_launch = elapsed / (launches reps);
+ * flops_per_launch (may be NULL) = the algorithmic 2 M N K of the timed
+ * transforms over the launches.  Per-launch event pairs
  * (mf_solver3d_profile) also count the host's enqueue gaps; this does not.
  * Synchronises the stream. */
 int mf_solver3d_time_transforms(mf_solver3d *sc, int precision, int reps, double *ms_per_launch,

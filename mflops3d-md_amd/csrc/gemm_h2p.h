@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gemm_h2.h"
+#include "lds_sync.h"
 
 namespace mf {
 
@@ -85,20 +86,6 @@ __device__ __forceinline__ h2p_frag h2p_read(const _Float16 *a, int off) {
   return h2p_frag{*reinterpret_cast<const f16x8_t *>(a), *reinterpret_cast<const f16x8_t *>(a + off)};
 }
 
-// Block barrier without a memory fence: __syncthreads() (a workgroup
-// release/acquire) also waits for every outstanding global load, direct
-// global-to-LDS loads included, which would drain the prefetched slices.
-// Here the wave's own LDS writes are waited for (lgkmcnt), a direct-to-LDS
-// load is waited for by an explicit vmcnt wait before the call, and empty
-// asm statements with a memory clobber keep the compiler from moving memory
-// accesses across the barrier.
-__device__ __forceinline__ void h2p_sync() {
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unconstrained
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 __device__ __forceinline__ float h2p_wave_max(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -129,9 +116,9 @@ __device__ __forceinline__ void h2p_phase1(f4_t (&acc)[A0][A1], float *__restric
   stage(1, stB);
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_waitcnt(h2p_vmcnt(16));  // slice 0 landed
-  h2p_sync();
+  lds_barrier();
   split(stA);
-  h2p_sync();
+  lds_barrier();
   // unrolled, so that the fragment buffers alternate without copies and the
   // compiler's own waits for them count the loads issued since
 #pragma unroll
@@ -171,13 +158,13 @@ __device__ __forceinline__ void h2p_phase1(f4_t (&acc)[A0][A1], float *__restric
       // remain in flight
       if (t + 2 < NK) __builtin_amdgcn_s_waitcnt(h2p_vmcnt(16));
       else __builtin_amdgcn_s_waitcnt(h2p_vmcnt(0));
-      h2p_sync();  // S consumed by every wave, slice t + 1 visible
+      lds_barrier();  // S consumed by every wave, slice t + 1 visible
       split(((t + 1) & 1) ? stB : stA);
-      h2p_sync();
+      lds_barrier();
     }
   }
   __builtin_amdgcn_s_waitcnt(0);
-  h2p_sync();  // the staging buffers are free for phase 2
+  lds_barrier();  // the staging buffers are free for phase 2
 }
 
 // Forward pair on planes of R[x + n1 (y + n2 z)] (FP32, largest magnitude
@@ -280,7 +267,7 @@ __global__ void __launch_bounds__(512, 1)
     // ly = 32 c + 16 nj + 4 g + r, kx = 32 wave + 16 mi + l%16
     h2p_stage_chunk<KPY>(Wyp, 0, stA, wave, lane);
     __builtin_amdgcn_s_waitcnt(0);
-    h2p_sync();
+    lds_barrier();
     float *Uz = U + z * plane;
     auto chunk = [&](int c, float *cur, float *nxt) {
       if (c + 1 < ncy) h2p_stage_chunk<KPY>(Wyp, c + 1, nxt, wave, lane);
@@ -314,7 +301,7 @@ __global__ void __launch_bounds__(512, 1)
         }
       }
       __builtin_amdgcn_s_waitcnt(0);
-      h2p_sync();  // chunk c consumed by every wave, chunk c + 1 landed
+      lds_barrier();  // chunk c consumed by every wave, chunk c + 1 landed
       if (active) {
 #pragma unroll
         for (int nj = 0; nj < 2; ++nj)
@@ -451,7 +438,7 @@ __global__ void __launch_bounds__(512, 1)
     // y = 32 wave + 16 nj + 4 g + r, x = 32 c + 16 xb + l%16
     h2p_stage_chunk<KPX>(Vxp, 0, stA, wave, lane);
     __builtin_amdgcn_s_waitcnt(0);
-    h2p_sync();
+    lds_barrier();
     double *uz = u + 1 + nx + nxy * (z + 1);
     float *uold = sbuf + wave * 32 * H2P_UP;
     auto chunk = [&](int c, float *cur, float *nxt) {
@@ -498,7 +485,7 @@ __global__ void __launch_bounds__(512, 1)
         }
       }
       __builtin_amdgcn_s_waitcnt(0);
-      h2p_sync();  // chunk c consumed by every wave, chunk c + 1 landed
+      lds_barrier();  // chunk c consumed by every wave, chunk c + 1 landed
       if (active) {
 #pragma unroll
         for (int nj = 0; nj < 2; ++nj)

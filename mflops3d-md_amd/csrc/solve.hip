@@ -1076,7 +1076,9 @@ mf::BandRows rows(const mf::Band &b) { return mf::BandRows{b.c0, b.w}; }
 // Back-to-back launches of the first five transforms of a solve (x, y, z
 // with the divide, z and y backward) on the handle's own work arrays, reps
 // times, between two events: the kernels' own durations, without the
-// host-side gaps that per-launch event pairs add.
+// host-side gaps that per-launch event pairs add.  For the fp16x2 plane
+// pairs: the three launches of an inner solve (x+y forward pair, z pair,
+// y+x backward pair written into a scratch field).
 extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int reps,
                                            double *ms_per_launch, double *flops_per_launch,
                                            void *stream) {
@@ -1085,6 +1087,12 @@ extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int re
   if (precision < MF_FP64 || precision > MF_MIXED_FP16X2)
     return mf::fail(MF_ERR_ARG, "mf_solver3d_time_transforms: unknown precision");
   hipStream_t st = (hipStream_t)stream;
+  const bool planes = precision == MF_MIXED_FP16X2 && h2_planes(h);
+  double *uscr = nullptr;  // the backward pair's target
+  if (planes) {
+    const size_t bytes = (size_t)h->nx * h->ny * h->nz * sizeof(double);
+    HIPCHK(hipMalloc(&uscr, bytes));
+  }
   const bool prof = h->prof;
   h->prof = false;
   hipEvent_t e0, e1;
@@ -1100,10 +1108,12 @@ extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int re
     else if (precision == MF_MIXED_BF16X3)
       transforms_bf3(h, h->R32, h->F1, h->F2, 0.0, nullptr, false, st, false);
     else
-      transforms_h2(h, h->R32, h->F1, h->F2, 0.0, nullptr, false, st, false);
+      transforms_h2(h, h->R32, h->F1, h->F2, 0.0, uscr, false, st, false);
   };
-  // launches per pass: five, or four when the fp16 z pair is one kernel
-  const int launches = (precision == MF_MIXED_FP16X2 && h2_zfused(h)) ? 4 : 5;
+  // launches per pass: five, four when the fp16 z pair is one kernel, three
+  // (or four, with separate z transforms) for the plane pairs
+  const int launches = planes ? (h2_zfused(h) ? 3 : 4)
+                              : (precision == MF_MIXED_FP16X2 && h2_zfused(h)) ? 4 : 5;
   five();  // warm
   hipError_t e = hipEventRecord(e0, st);
   for (int r = 0; r < reps && e == hipSuccess; ++r) five();
@@ -1113,13 +1123,14 @@ extern "C" int mf_solver3d_time_transforms(mf_solver3d *h, int precision, int re
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (uscr) (void)hipFree(uscr);
   h->prof = prof;
   if (e != hipSuccess)
     return mf::fail(MF_ERR_DEVICE, std::string("mf_solver3d_time_transforms: ") + hipGetErrorString(e));
   *ms_per_launch = ms / ((double)launches * reps);
-  if (flops_per_launch)  // 2 M N K per transform, K = n1, n2, n3, n3, n2 in turn
+  if (flops_per_launch)  // 2 M N K per transform, K = n1, n2, n3, n3, n2 (, n1) in turn
     *flops_per_launch = 2.0 * h->n1 * h->n2 * h->n3 *
-                        (h->n1 + 2.0 * h->n2 + 2.0 * h->n3) / launches;
+                        ((planes ? 2.0 : 1.0) * h->n1 + 2.0 * h->n2 + 2.0 * h->n3) / launches;
   return MF_OK;
 }
 

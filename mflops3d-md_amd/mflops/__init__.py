@@ -287,8 +287,9 @@ class SolverCoeffs3D:
 
     def time_transforms(self, precision: int, reps: int = 10, stream=None):
         """(ms per launch, algorithmic flops per launch) of the first five
-        transforms of a solve launched back to back (overwrites the handle's
-        work arrays only)."""
+        transforms of a solve launched back to back, or, for the fp16x2
+        plane pairs, of a whole inner solve's launches (overwrites the
+        handle's work arrays only)."""
         ms, fl = C.c_double(), C.c_double()
         check(self._lib.mf_solver3d_time_transforms(self._h, int(precision), int(reps),
                                                     C.byref(ms), C.byref(fl), _stream(stream)))
