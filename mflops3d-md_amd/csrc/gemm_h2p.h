@@ -22,15 +22,17 @@
 // copied to one of two FP32 staging buffers with direct global-to-LDS loads
 // (two slices in flight), split cooperatively into two fp16 parts, and
 // multiplied by the wave's own rows of the first matrix (fragments from
-// global memory, one slice ahead).  Phase 2 (second product), per 32-row
+// global memory, one slice ahead); each wave splits the part it staged, into
+// one of two split slices, so a slice takes one block barrier.  Phase 2 (second product), per 32-row
 // chunk of the second matrix: the chunk (already split, k permuted) is
 // copied to LDS one chunk ahead, the wave's 32 x 32 outputs are formed and
 // stored while the next chunk's products run.  Inside both loops each
 // step's LDS fragments are read one step ahead.
 //
-// LDS: two staging buffers of 32 KB (phase 1: field slices; phase 2: matrix
-// chunks) and a third array holding the split slice S[2][256][40] (phase 1)
-// and, for the backward add, each wave's old values of u (phase 2).  They
+// LDS (144 KB, one block per CU): two staging buffers of 32 KB (phase 1:
+// field slices; phase 2: matrix chunks) and a third array holding two split
+// slices S[2][256][40] (phase 1) and, for the backward add, each wave's old
+// values of u (phase 2).  They
 // are separate arrays, with the buffer of every access fixed at compile time
 // (loops unrolled by buffer parity), so that the compiler's waits before an
 // LDS read cover only the direct global-to-LDS loads into that array.
@@ -45,12 +47,13 @@ namespace mf {
 constexpr int H2P_N = 256;             // largest n1, n2 (8 waves x 32 rows)
 constexpr int H2P_SP = 40;             // split slice row pitch (halves): 80-byte rows
 constexpr int H2P_STAGE = 32 * H2P_N;  // FP32 values per staging buffer (32 KB)
-constexpr int H2P_SLDS = 2 * H2P_N * H2P_SP / 2;  // the split slice (floats)
+constexpr int H2P_SLDS = 2 * H2P_N * H2P_SP / 2;  // one split slice (floats)
 // backward add: each wave's 32 x 32 old values of u (FP64) per x chunk, rows
 // of 32 doubles at a pitch of 72 floats (the four row groups a wave reads at
 // once fall on disjoint banks)
 constexpr int H2P_UP = 72;
 constexpr int H2P_ULDS = 8 * 32 * H2P_UP;
+static_assert(H2P_ULDS <= 2 * H2P_SLDS, "the old values of u fit the two split slices");
 
 // contraction index held at position qq (0..31) of a permuted 32-wide k-step
 __host__ __device__ constexpr int h2p_perm(int qq) {
@@ -96,31 +99,37 @@ __device__ __forceinline__ float h2p_wave_max(float v) {
 // field, of the split slice's NB 16-row blocks (fragments from S rows
 // 16 i + l%16) with the wave's fragments of the first matrix.  stage(t, buf)
 // issues slice t's field loads (16 wave instructions) into a staging
-// buffer, split(buf) splits a staged slice into S, load(t, frag) loads the
-// wave's matrix fragments of slice t (4 wave instructions).  The explicit
-// waits for a staged slice count only the 16 field loads of the slice staged
-// after it as younger: the compiler may move the fragment loads, and waiting
-// for them too is cheap, while counting them and finding them moved past the
-// wait would leave the slice's last loads in flight.  The block index i goes
-// to acc's first index when ROWS_FIRST, else to its second.
+// buffer; split(buf, S) splits the part of a staged slice that this wave
+// loaded itself into a split slice; load(t, frag) loads the wave's matrix
+// fragments of slice t (4 wave instructions).  Each wave splits only what it
+// staged, so a slice needs no barrier before its split, only the wave's own
+// wait: its explicit vmcnt counts only the 16 field loads of the slice
+// staged after it as younger (the compiler may move the fragment loads;
+// waiting for them too is cheap, while counting them and finding them moved
+// past the wait would leave the slice's last loads in flight).  The split
+// slices alternate between two buffers, so one barrier per slice orders
+// both the split before its products and the products before the next
+// split into the same buffer.  The block index i goes to acc's first index
+// when ROWS_FIRST, else to its second.
 template <int NK, int NB, bool ROWS_FIRST, int A0, int A1, typename Stage, typename Split,
           typename Load>
 __device__ __forceinline__ void h2p_phase1(f4_t (&acc)[A0][A1], float *__restrict__ stA,
                                            float *__restrict__ stB, float *__restrict__ sbuf,
                                            bool active, int rl, int g, Stage stage, Split split,
                                            Load load) {
-  _Float16(*S)[H2P_N][H2P_SP] = reinterpret_cast<_Float16(*)[H2P_N][H2P_SP]>(sbuf);
+  _Float16 *S0 = reinterpret_cast<_Float16 *>(sbuf);
+  _Float16 *S1 = reinterpret_cast<_Float16 *>(sbuf + H2P_SLDS);
   f16x8_t wf[2][2][2];  // [slice parity][part][16-row block]
   load(0, wf[0]);
   stage(0, stA);
   stage(1, stB);
   asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(h2p_vmcnt(16));  // slice 0 landed
+  __builtin_amdgcn_s_waitcnt(h2p_vmcnt(16));  // this wave's part of slice 0 landed
+  asm volatile("" ::: "memory");
+  split(stA, S0);
   lds_barrier();
-  split(stA);
-  lds_barrier();
-  // unrolled, so that the fragment buffers alternate without copies and the
-  // compiler's own waits for them count the loads issued since
+  // unrolled, so that the buffers alternate without copies and the
+  // compiler's own waits for the fragments count the loads issued since
 #pragma unroll
   for (int t = 0; t < NK; ++t) {
     f16x8_t(&wc)[2][2] = wf[t & 1];
@@ -132,7 +141,7 @@ __device__ __forceinline__ void h2p_phase1(f4_t (&acc)[A0][A1], float *__restric
     if (active) {
       // every 16-row block of S: blocks past the field's extent hold copies
       // of its last row, which meet zero columns of the second matrix
-      const _Float16 *Sa = &S[0][rl][8 * g];
+      const _Float16 *Sa = ((t & 1) ? S1 : S0) + rl * H2P_SP + 8 * g;
       constexpr int SOFF = H2P_N * H2P_SP;  // part 1 - part 0, halves
       h2p_frag a = h2p_read(Sa, SOFF);
 #pragma unroll
@@ -154,13 +163,13 @@ __device__ __forceinline__ void h2p_phase1(f4_t (&acc)[A0][A1], float *__restric
       }
     }
     if (t + 1 < NK) {
-      // slice t + 1 landed: only slice t + 2's field loads (if any) may
-      // remain in flight
+      // this wave's part of slice t + 1 landed: only slice t + 2's field
+      // loads (if any) may remain in flight
       if (t + 2 < NK) __builtin_amdgcn_s_waitcnt(h2p_vmcnt(16));
       else __builtin_amdgcn_s_waitcnt(h2p_vmcnt(0));
-      lds_barrier();  // S consumed by every wave, slice t + 1 visible
-      split(((t + 1) & 1) ? stB : stA);
-      lds_barrier();
+      asm volatile("" ::: "memory");
+      split(((t + 1) & 1) ? stB : stA, ((t + 1) & 1) ? S1 : S0);
+      lds_barrier();  // slice t + 1 split; every wave is past slice t - 1's products
     }
   }
   __builtin_amdgcn_s_waitcnt(0);
@@ -180,9 +189,9 @@ __global__ void __launch_bounds__(512, 1)
                    const _Float16 *__restrict__ Wx, const _Float16 *__restrict__ Wyp, float wsx,
                    float wsy, float *__restrict__ U, const unsigned *__restrict__ fmax_in,
                    unsigned *__restrict__ fmax_out) {
-  __shared__ __attribute__((aligned(16))) float stA[H2P_STAGE], stB[H2P_STAGE], sbuf[H2P_SLDS];
+  __shared__ __attribute__((aligned(16))) float stA[H2P_STAGE], stB[H2P_STAGE],
+      sbuf[2 * H2P_SLDS];
   __shared__ float bmax[8];
-  _Float16(*S)[H2P_N][H2P_SP] = reinterpret_cast<_Float16(*)[H2P_N][H2P_SP]>(sbuf);
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, rl = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const float fs = h2_scale(__uint_as_float(*fmax_in));
@@ -197,7 +206,8 @@ __global__ void __launch_bounds__(512, 1)
   {
     const int z = blockIdx.x;
     const float *Rz = R + z * plane;
-    // field slice t: x in [32 t, 32 t + 32), y in [0, 256) -> buf[y][32]
+    // field slice t: x in [32 t, 32 t + 32), y in [0, 256) -> buf[y][32];
+    // wave w stages rows y = 2 i, 2 i + 1 of its instructions i = w + 8 q
     auto stage = [&](int t, float *buf) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
@@ -206,8 +216,10 @@ __global__ void __launch_bounds__(512, 1)
             Rz + (long)(y < n2 ? y : n2 - 1) * n1 + (x < n1 ? x : n1 - 1), buf + 64 * i, 4, 0, 0);
       }
     };
-    auto split = [&](const float *buf) {
-      const int y = tid >> 1, x0 = 16 * (tid & 1);
+    // this wave's 32 staged rows: lane l takes 16 x of row 2 i + h, i = wave
+    // + 8 (l / 4), h = (l / 2) % 2, x0 = 16 (l % 2)
+    auto split = [&](const float *buf, _Float16 *Sd) {
+      const int i = wave + 8 * (lane >> 2), y = 2 * i + ((lane >> 1) & 1), x0 = 16 * (lane & 1);
       f16x8_t p0[2], p1[2];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
@@ -218,8 +230,8 @@ __global__ void __launch_bounds__(512, 1)
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        *reinterpret_cast<f16x8_t *>(&S[0][y][x0 + 8 * h]) = p0[h];
-        *reinterpret_cast<f16x8_t *>(&S[1][y][x0 + 8 * h]) = p1[h];
+        *reinterpret_cast<f16x8_t *>(Sd + y * H2P_SP + x0 + 8 * h) = p0[h];
+        *reinterpret_cast<f16x8_t *>(Sd + H2P_N * H2P_SP + y * H2P_SP + x0 + 8 * h) = p1[h];
       }
     };
     auto load = [&](int t, f16x8_t (&wf)[2][2]) {
@@ -351,8 +363,7 @@ __global__ void __launch_bounds__(512, 1)
                    float wsx, double *__restrict__ u, int nx, int ny,
                    const unsigned *__restrict__ fmax_in, unsigned *__restrict__ zero_slot) {
   __shared__ __attribute__((aligned(16))) float stA[H2P_STAGE], stB[H2P_STAGE],
-      sbuf[ADD ? H2P_ULDS : H2P_SLDS];
-  _Float16(*S)[H2P_N][H2P_SP] = reinterpret_cast<_Float16(*)[H2P_N][H2P_SP]>(sbuf);
+      sbuf[2 * H2P_SLDS];
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, rl = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (zero_slot && blockIdx.x == 0 && tid == 0) *zero_slot = 0u;
@@ -368,30 +379,34 @@ __global__ void __launch_bounds__(512, 1)
   {
     const int z = blockIdx.x;
     const float *Vz = V + z * plane;
-    // field slice t: ly in [32 t, 32 t + 32), kx in [0, 256) -> buf[ly][256]
+    // field slice t: ly in [32 t, 32 t + 32), kx in [0, 256) -> buf[ly][256];
+    // wave w stages the four rows ly = 32 t + 4 w .. 4 w + 3
     auto stage = [&](int t, float *buf) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int i = wave + 8 * q, ly = 32 * t + (i >> 2), kx = 64 * (i & 3) + lane;
+        const int i = 16 * wave + q, ly = 32 * t + (i >> 2), kx = 64 * (i & 3) + lane;
         __builtin_amdgcn_global_load_lds(
             Vz + (long)(ly < n2 ? ly : n2 - 1) * n1 + (kx < n1 ? kx : n1 - 1), buf + 64 * i, 4, 0,
             0);
       }
     };
-    auto split = [&](const float *buf) {
-      const int kx = tid & 255, l0 = 16 * (tid >> 8);
-      f16x8_t p0[2], p1[2];
+    // this wave's four staged rows: lane l takes kx = l + 64 c (c < 4), the
+    // four ly of each into S[part][kx][4 wave .. 4 wave + 3]
+    typedef _Float16 f16x4_t __attribute__((ext_vector_type(4)));
+    auto split = [&](const float *buf, _Float16 *Sd) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        _Float16 a, b;
-        split2h(buf[H2P_N * (l0 + q) + kx] * fv, a, b);
-        p0[q >> 3][q & 7] = a;
-        p1[q >> 3][q & 7] = b;
-      }
+      for (int c = 0; c < 4; ++c) {
+        const int kx = lane + 64 * c;
+        f16x4_t p0, p1;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        *reinterpret_cast<f16x8_t *>(&S[0][kx][l0 + 8 * h]) = p0[h];
-        *reinterpret_cast<f16x8_t *>(&S[1][kx][l0 + 8 * h]) = p1[h];
+        for (int r = 0; r < 4; ++r) {
+          _Float16 a, b;
+          split2h(buf[H2P_N * (4 * wave + r) + kx] * fv, a, b);
+          p0[r] = a;
+          p1[r] = b;
+        }
+        *reinterpret_cast<f16x4_t *>(Sd + kx * H2P_SP + 4 * wave) = p0;
+        *reinterpret_cast<f16x4_t *>(Sd + H2P_N * H2P_SP + kx * H2P_SP + 4 * wave) = p1;
       }
     };
     auto load = [&](int t, f16x8_t (&vf)[2][2]) {
