@@ -531,4 +531,233 @@ __global__ void __launch_bounds__(512, 1)
   }
 }
 
+// The z transforms of an inner solve with the divide between them, on
+// columns: out[n + n12 z] = sum_kz Vz(z, kz) T(kz, n) with
+// T(kz, n) = sum_z' Wz(kz, z') F[n + n12 z'] / (lx(i1) + ly(i2) + lz(kz) +
+// sigma), n = i1 + n1 i2, and T = 0 where |lx|, |ly|, |lz| and |sigma| are
+// all below 1e-11 (the null mode; class_solver_3d.f90:202-225).  A block of 8
+// waves owns 256 consecutive columns n, each wave 32 of them with every kz of
+// T in 128 accumulator VGPRs, so both contractions run over indices the wave
+// holds: the field's values are loaded straight into operand fragments
+// (registers, one z-slice ahead), and only the two z matrices, which every
+// wave needs whole, stream through LDS (32-wide z-slices of Wz, then 32-row
+// chunks of Vz with k permuted as in the plane pairs).  F's largest magnitude
+// is in *fmax_in; the largest stored magnitude goes to *fmax_out.  Wz: split
+// [2][KPZ/8][256][8] (rows kz, scaled by wsw), Vzp: split [2][KPZ/8][256][8]
+// (rows z, k permuted, scaled by wsv).  grid: ceil(n12 / 256) blocks of 512
+// threads; n3 <= KPZ, KPZ 192 or 256.
+template <int KPZ>
+__global__ void __launch_bounds__(512, 1)
+    h2c_kernel(const float *__restrict__ F, long n12, int n3, const _Float16 *__restrict__ Wz,
+               const _Float16 *__restrict__ Vzp, float wsw, float wsv, float *__restrict__ out,
+               EpiArgs ep, const unsigned *__restrict__ fmax_in, unsigned *__restrict__ fmax_out) {
+  __shared__ __attribute__((aligned(16))) float stA[H2P_STAGE], stB[H2P_STAGE];
+  __shared__ __attribute__((aligned(16))) float lzs[H2P_N];
+  __shared__ unsigned char lzn[H2P_N];
+  __shared__ float bmax[8];
+  constexpr int NK = KPZ / 32;  // z-slices of the first product, k-steps of the second
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, rl = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const long n0 = 256L * blockIdx.x + 32 * wave;  // this wave's first column
+  const bool active = n0 < n12;
+  const float fs = h2_scale(__uint_as_float(*fmax_in));
+  const float u1 = 1.0f / (fs * wsw);
+  const int ncz = (n3 + 31) / 32;
+  for (int k = tid; k < H2P_N; k += 512) {
+    const double l = k < n3 ? ep.lz[k] : 1.0;
+    lzs[k] = (float)l;
+    lzn[k] = fabs(l) < 1e-11 ? 1 : 0;
+  }
+  // this lane's two columns n = n0 + 16 nj + l%16 (clamped for the loads)
+  long nc[2];
+  float lxy[2];
+  bool nxy[2];
+#pragma unroll
+  for (int nj = 0; nj < 2; ++nj) {
+    const long n = n0 + 16 * nj + rl;
+    nc[nj] = n < n12 ? n : n12 - 1;
+    const int i1 = (int)(nc[nj] % ep.n1), i2 = (int)(nc[nj] / ep.n1);
+    lxy[nj] = (float)(ep.lx[i1] + ep.ly[i2] + ep.sigma);
+    nxy[nj] = fmax(fmax(fabs(ep.lx[i1]), fabs(ep.ly[i2])), fabs(ep.sigma)) < 1e-11;
+  }
+  // the field values of z-slice t this lane's fragments hold: z = 32 t + 8 g + q
+  auto load_f = [&](int t, float (&fv)[2][8]) {
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int z = 32 * t + 8 * g + q;
+        fv[nj][q] = F[nc[nj] + n12 * (z < n3 ? z : n3 - 1)];
+      }
+  };
+  // z-slice t of Wz: [2][4][256][8] halves, 128 wave loads of 256 bytes
+  auto stage_w = [&](int t, float *buf) {
+    const float *Wf = reinterpret_cast<const float *>(Wz);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = wave + 8 * q, pg = i >> 4, h = i & 15;  // pg = 4 part + group
+      __builtin_amdgcn_global_load_lds(
+          Wf + ((long)((pg >> 2) * (KPZ / 8) + 4 * t + (pg & 3)) * H2P_N) * 4 + 64 * h + lane,
+          buf + 64 * i, 4, 0, 0);
+    }
+  };
+
+  // ---- phase 1: T(kz, n); lane holds kz = 16 mi + 4 g + r, n = n0 + 16 nj + l%16
+  f4_t T[16][2];
+#pragma unroll
+  for (int mi = 0; mi < 16; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj) T[mi][nj] = (f4_t){0, 0, 0, 0};
+  float fv[2][2][8];  // [slice parity][nj][q]
+  load_f(0, fv[0]);
+  stage_w(0, stA);
+  __builtin_amdgcn_s_waitcnt(0);
+  lds_barrier();
+#pragma unroll
+  for (int t = 0; t < NK; ++t) {
+    const float *buf = (t & 1) ? stB : stA;
+    if (t + 1 < NK) {
+      stage_w(t + 1, (t & 1) ? stA : stB);
+      load_f(t + 1, fv[(t + 1) & 1]);
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    if (active) {
+      f16x8_t ff[2][2];  // [nj][part]
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          _Float16 a, b;
+          split2h(fv[t & 1][nj][q] * fs, a, b);
+          ff[nj][0][q] = a;
+          ff[nj][1][q] = b;
+        }
+      const _Float16 *Wa = reinterpret_cast<const _Float16 *>(buf) + (g * H2P_N + rl) * 8;
+      constexpr int WOFF = 4 * H2P_N * 8;  // part 1 - part 0, halves
+      h2p_frag a = h2p_read(Wa, WOFF);
+#pragma unroll
+      for (int mi = 0; mi < KPZ / 16; ++mi) {
+        h2p_frag an = a;
+        if (mi + 1 < KPZ / 16) an = h2p_read(Wa + 16 * (mi + 1) * 8, WOFF);
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj) {
+          f4_t c = T[mi][nj];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.p1, ff[nj][0], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.p0, ff[nj][1], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.p0, ff[nj][0], c, 0, 0, 0);
+          T[mi][nj] = c;
+        }
+        a = an;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // slice t + 1 (matrix in LDS from every wave, field in registers) landed
+    __builtin_amdgcn_s_waitcnt(0);
+    lds_barrier();
+  }
+
+  // ---- the divide, then the wave's slab scaled into fp16 range and split
+  float tm = 0.0f;
+#pragma unroll
+  for (int mi = 0; mi < 16; ++mi) {
+    const int kz0 = 16 * mi + 4 * g;
+    const f4_t lz4 = *reinterpret_cast<const f4_t *>(&lzs[kz0]);
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool nul = nxy[nj] && lzn[kz0 + r];
+        const float v = nul ? 0.0f : T[mi][nj][r] * __builtin_amdgcn_rcpf(lxy[nj] + lz4[r]);
+        T[mi][nj][r] = v;
+        tm = fmaxf(tm, fabsf(v));
+      }
+  }
+  const float sT = h2_scale(h2p_wave_max(tm));
+  const float u2 = 1.0f / (sT * wsv);
+  f16x8_t ts[2][NK][2];  // [nj][k-step][part]
+#pragma unroll
+  for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+    for (int j = 0; j < NK; ++j)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        _Float16 a, b;
+        split2h((q < 4 ? T[2 * j][nj][q] : T[2 * j + 1][nj][q - 4]) * sT, a, b);
+        ts[nj][j][0][q] = a;
+        ts[nj][j][1][q] = b;
+      }
+
+  // ---- phase 2: out(z, n) = sum_kz Vz(z, kz) T(kz, n); lane holds
+  // z = 32 c + 16 zb + 4 g + r, n = n0 + 16 nj + l%16
+  float amax = 0.0f;
+  h2p_stage_chunk<KPZ>(Vzp, 0, stA, wave, lane);
+  __builtin_amdgcn_s_waitcnt(0);
+  lds_barrier();
+  auto chunk = [&](int c, float *cur, float *nxt) {
+    if (c + 1 < ncz) h2p_stage_chunk<KPZ>(Vzp, c + 1, nxt, wave, lane);
+    f4_t acc[2][2];  // [zb][nj]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = (f4_t){0, 0, 0, 0};
+    if (active) {
+      const _Float16 *Wa = reinterpret_cast<const _Float16 *>(cur) + (g * 32 + rl) * 8;
+      constexpr int WOFF = (KPZ / 8) * 32 * 8;
+      h2p_frag w = h2p_read(Wa, WOFF);
+#pragma unroll
+      for (int st = 0; st < 2 * NK; ++st) {
+        const int j = st >> 1, zb = st & 1;
+        h2p_frag wn = w;
+        if (st + 1 < 2 * NK)
+          wn = h2p_read(Wa + (4 * ((st + 1) >> 1) * 32 + 16 * ((st + 1) & 1)) * 8, WOFF);
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj) {
+          f4_t v = acc[zb][nj];
+          v = __builtin_amdgcn_mfma_f32_16x16x32_f16(w.p1, ts[nj][j][0], v, 0, 0, 0);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_f16(w.p0, ts[nj][j][1], v, 0, 0, 0);
+          v = __builtin_amdgcn_mfma_f32_16x16x32_f16(w.p0, ts[nj][j][0], v, 0, 0, 0);
+          acc[zb][nj] = v;
+        }
+        w = wn;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    lds_barrier();  // chunk c consumed by every wave, chunk c + 1 landed
+    if (active) {
+#pragma unroll
+      for (int zb = 0; zb < 2; ++zb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int z = 32 * c + 16 * zb + 4 * g + r;
+          if (z >= n3) continue;
+#pragma unroll
+          for (int nj = 0; nj < 2; ++nj) {
+            const long n = n0 + 16 * nj + rl;
+            if (n >= n12) continue;
+            const float v = acc[zb][nj][r] * u1 * u2;
+            amax = fmaxf(amax, fabsf(v));
+            out[n + n12 * z] = v;
+          }
+        }
+    }
+  };
+  for (int c = 0; c < ncz; c += 2) {
+    chunk(c, stA, stB);
+    if (c + 1 < ncz) chunk(c + 1, stB, stA);
+  }
+  if (fmax_out) {
+    const float vmax = h2p_wave_max(amax);
+    if (lane == 0) bmax[wave] = vmax;
+    __syncthreads();
+    if (tid == 0) {
+      float b = bmax[0];
+#pragma unroll
+      for (int q = 1; q < 8; ++q) b = fmaxf(b, bmax[q]);
+      atomicMax(fmax_out, __float_as_uint(b));
+    }
+  }
+}
+
 }  // namespace mf

@@ -212,7 +212,7 @@ struct mf_solver3d {
   float wsc[3][2] = {{1, 1}, {1, 1}, {1, 1}};
   // the forward y and backward x fp16 splits again with k permuted within
   // each 32-wide k-step (h2p_perm), the second operands of the plane pairs
-  _Float16 *Whp1 = nullptr, *Vhp0 = nullptr;
+  _Float16 *Whp1 = nullptr, *Vhp0 = nullptr, *Vhp2 = nullptr;
   // largest magnitudes (float bits) of the fp16 path's fields: [0] r' / b',
   // [1..5] the outputs of the first five transforms
   unsigned *fmx = nullptr;
@@ -239,6 +239,7 @@ struct mf_solver3d {
   // n3 plane partials
   bool h2_resident = true;  // fp16x2 transforms with the eigenvectors in registers (MF_H2_RESIDENT=0: off)
   bool h2_plane = true;     // fp16x2 x/y transform pairs per z-plane (MF_H2_PLANE=0: off)
+  bool h2_cols = true;      // fp16x2 z pair on column blocks (MF_H2_COLS=0: the row-tile kernel)
   int ncu = 256;        // compute units of the device the handle was created on
   bool null_mode = false;
   double *w0 = nullptr, *npart = nullptr;
@@ -534,6 +535,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
   }
   if (const char *env = std::getenv("MF_H2_RESIDENT")) h->h2_resident = std::atoi(env) != 0;
   if (const char *env = std::getenv("MF_H2_PLANE")) h->h2_plane = std::atoi(env) != 0;
+  if (const char *env = std::getenv("MF_H2_COLS")) h->h2_cols = std::atoi(env) != 0;
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -607,6 +609,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     }
     h->Whp1 = (_Float16 *)take((2 * (size_t)h->wrows[1] * h->kp[1] + 3) / 4);
     h->Vhp0 = (_Float16 *)take((2 * (size_t)h->wrows[0] * h->kp[0] + 3) / 4);
+    h->Vhp2 = (_Float16 *)take((2 * (size_t)h->wrows[2] * h->kp[2] + 3) / 4);
     h->fmx = (unsigned *)take(4);
     h->w0 = take((size_t)n[0] + n[1] + n[2]);
     h->npart = take(n[2]);
@@ -675,7 +678,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
         }
       if (e == hipSuccess)
         e = up(which == 0 ? (void *)h->Whs[a] : (void *)h->Vhs[a], hp.data(), hp.size() * 2);
-      if (e == hipSuccess && ((a == 1 && which == 0) || (a == 0 && which == 1))) {
+      if (e == hipSuccess && ((a == 1 && which == 0) || (a != 1 && which == 1))) {
         // k permuted within each 32-wide k-step: position kk holds k = h2p_perm(kk)
         std::vector<uint16_t> pp(hp.size(), 0);
         for (int part = 0; part < 2; ++part)
@@ -685,7 +688,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
               pp[(((size_t)part * (kp / 8) + kk / 8) * wr + r) * 8 + kk % 8] =
                   hp[(((size_t)part * (kp / 8) + k / 8) * wr + r) * 8 + k % 8];
           }
-        e = up(a == 1 ? (void *)h->Whp1 : (void *)h->Vhp0, pp.data(), pp.size() * 2);
+        e = up(a == 1 ? (void *)h->Whp1 : a == 0 ? (void *)h->Vhp0 : (void *)h->Vhp2, pp.data(),
+               pp.size() * 2);
       }
     }
   }
@@ -992,9 +996,25 @@ void launch_h2p_bwd(mf_solver3d *h, const float *in, double *u, bool accumulate,
   else launch_h2p_bwd_a<false>(h, in, u, st);
 }
 
+template <int KPZ>
+void launch_h2c(mf_solver3d *h, const float *in, float *out, const EpiArgs &ep, hipStream_t st) {
+  const long n12 = (long)h->n1 * h->n2;
+  hipLaunchKernelGGL((mf::h2c_kernel<KPZ>), dim3((unsigned)((n12 + 255) / 256)), dim3(512), 0, st,
+                     in, n12, h->n3, (const _Float16 *)h->Whs[2], (const _Float16 *)h->Vhp2,
+                     h->wsc[2][0], h->wsc[2][1], out, ep, (const unsigned *)(h->fmx + 2),
+                     h->fmx + 4);
+}
+
+// the fused z pair: column blocks (gemm_h2p.h h2c_kernel) when k is padded to
+// 192 or 256, else row tiles (gemm_h2z_kernel)
 void launch_h2z(mf_solver3d *h, const float *in, float *out, const EpiArgs &ep, hipStream_t st) {
   const long n12 = (long)h->n1 * h->n2;
   ProfScope ps(h, P_T32, 4.0 * n12 * h->n3 * h->n3, st);
+  if (h->h2_cols && (h->kp[2] == 192 || h->kp[2] == 256)) {
+    if (h->kp[2] == 256) launch_h2c<256>(h, in, out, ep, st);
+    else launch_h2c<192>(h, in, out, ep, st);
+    return;
+  }
   hipLaunchKernelGGL(mf::gemm_h2z_kernel, dim3((unsigned)((n12 + mf::H2Z_BM - 1) / mf::H2Z_BM)),
                      dim3(256), 0, st, in, n12, (const _Float16 *)h->Whs[2],
                      (const _Float16 *)h->Vhs[2], h->kp[2], h->wrows[2], out, n12, (int)n12, h->n3,

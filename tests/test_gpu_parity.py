@@ -287,14 +287,18 @@ def test_solve_mixed_residual_kernels(monkeypatch, cz, shape):
         assert rep.residual <= 1e-13 and true_res <= 2e-13
 
 
-@pytest.mark.parametrize("shape", [(140, 200, 136), (258, 150, 130)])
-def test_solve_fp16x2_plane_pairs(monkeypatch, shape):
-    """The fp16x2 x/y transform pairs per z-plane (x and y of 129..256
-    points, k padded to 192 and to 256; 134 planes with the fused z pair,
-    128 planes with separate z transforms) against the separate transforms
-    (MF_H2_PLANE=0) and the FP64 solve: the residual contract checked by the
-    80-bit residual, 2e-11 against FP64, and the two paths differing (so
-    that both ran)."""
+@pytest.mark.parametrize("shape,switch", [((140, 200, 136), "MF_H2_PLANE"),
+                                          ((258, 150, 130), "MF_H2_PLANE"),
+                                          ((140, 200, 136), "MF_H2_COLS"),
+                                          ((136, 70, 200), "MF_H2_COLS")])
+def test_solve_fp16x2_plane_pairs(monkeypatch, shape, switch):
+    """The fp16x2 x/y transform pairs per z-plane (MF_H2_PLANE; x and y of
+    129..256 points, k padded to 192 and to 256; 134 planes with the fused z
+    pair, 128 planes with separate z transforms) and the column-block z pair
+    (MF_H2_COLS; 134 and 198 z points, k padded to 192 and 256, the last
+    column block partial) against the kernels they replace (switch = 0) and
+    the FP64 solve: the residual contract checked by the 80-bit residual,
+    2e-11 against FP64 each, and the two paths differing (so that both ran)."""
     mf, torch = _mf()
     bct, sigma = (1, 2, 1, 1, 2, 1), -300.0
     nx, ny, nz = shape
@@ -310,8 +314,8 @@ def test_solve_fp16x2_plane_pairs(monkeypatch, shape):
     scale = np.abs(u64[inner]).max()
     got = {}
     for plane in ("1", "0"):
-        monkeypatch.setenv("MF_H2_PLANE", plane)
-        print("run fp16x2 plane pairs=%s shape=%s" % (plane, shape), flush=True)
+        monkeypatch.setenv(switch, plane)
+        print("run fp16x2 %s=%s shape=%s" % (switch, plane, shape), flush=True)
         u, rep = _solve_gpu(mf, grids, bct, g, bcs, sigma, MIXED["fp16x2"])
         res = _true_residual(sc, u, g, bcs, sigma)
         err = float(np.abs(u[inner] - u64[inner]).max() / scale)
@@ -320,8 +324,10 @@ def test_solve_fp16x2_plane_pairs(monkeypatch, shape):
         assert rep.residual <= 1e-12 and res <= 2e-12 and err <= 2e-11
         got[plane] = u
     diff = float(np.abs(got["1"][inner] - got["0"][inner]).max() / scale)
-    print("  plane pairs vs separate transforms: %.2e" % diff, flush=True)
-    assert 0.0 < diff <= 2e-11
+    print("  %s=1 vs 0: %.2e" % (switch, diff), flush=True)
+    # both within 2e-11 of FP64, so within 4e-11 of each other; nonzero:
+    # both kernels ran
+    assert 0.0 < diff <= 4e-11
 
 
 def test_solve_c1_helmholtz_and_poisson():
