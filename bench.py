@@ -212,10 +212,10 @@ def main():
                      "unit": "GB/s", "frac": round(ach / PEAK["hbm_gbs"], 4)}, **common)
 
     if planes:
-        tr = [measured_traffic(k) for k in ("h2p_fwd_kernel", "gemm_h2z_kernel", "h2p_bwd_kernel<store>")]
+        tr = [measured_traffic(k) for k in ("h2p_fwd_kernel", "h2c_kernel", "h2p_bwd_kernel<store>")]
         traf_h2 = sum(tr) / 3 if all(t is not None for t in tr) else None
         h2 = roof("fp16x2", PEAK["fp16_mfma_tflops"], 3.0, traf=traf_h2, bytes_per_point=28.0 / 3)
-        h2["kernels"] = "h2p_fwd_kernel, gemm_h2z_kernel, h2p_bwd_kernel (store), one launch each"
+        h2["kernels"] = "h2p_fwd_kernel, h2c_kernel (z pair), h2p_bwd_kernel (store), one launch each"
     else:
         h2 = roof("fp16x2", PEAK["fp16_mfma_tflops"], 3.0, traf=measured_traffic("gemm_h2_kernel"))
     roofs = {"fp32": roof("fp32", PEAK["fp32_mfma_tflops"],
