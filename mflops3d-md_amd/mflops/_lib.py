@@ -124,6 +124,15 @@ def load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libmflops.so not built: run `python mflops3d-md_amd/build.py` "
                            "(expected at %s)" % LIB_PATH)
+    # torch (which holds the device fields) brings a HIP runtime of its own:
+    # it has to initialise before this library's first HIP call, which would
+    # otherwise leave torch without a device
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

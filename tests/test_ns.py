@@ -359,6 +359,27 @@ def test_ns_steps_match_oracle(tou, top):
     assert abs(g - orc.flow.g) < 1e-10 * abs(orc.flow.g)
 
 
+@pytest.mark.gpu
+def test_ns_advective_form_matches_oracle():
+    """nlt=1 (advective nonlinear term) against the oracle."""
+    import mflops as mf
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((20, 24, 18))
+    re, dt, steps = 3000.0, 1e-3, 4
+    print("run ns nlt=1 shape=%s steps=%d" % (u.shape, steps), flush=True)
+    orc = ns.NSOracle(gx, gy, gz, re, dt, nlt=1)
+    orc.set_state(u, v, w)
+    for c in range(3):
+        orc.bc["uvw"[c]] = bc[c]
+    orc.flow = ns.FlowRate(g0=2.0 / re)
+    for _ in range(steps):
+        orc.step()
+    dev = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt, nlt=1)
+    dev.set_flow_rate(True, 2.0 / re)
+    dev.step(steps, mf.MF_FP64)
+    _compare(mf, dev, orc, 1e-10, "nlt=1")
+
+
 def _orc_fields(orc):
     return list(orc.velocity()) + [orc.pressure(), orc.phi_latest()]
 
