@@ -18,8 +18,9 @@ and the largest single-kernel share of the solve; its average launch time
 from HIP event pairs around each launch in a second pass of the same
 solves), the roofline of its transform kernels (transform_roofline: the
 launches of an inner solve's transforms back to back between two HIP
-events) and the CPU baseline (the FP64 oracle on the host's cores, a bounded sample of
-the same workload).  Extra keys: poisson_sigma0 (the same problem at sigma = 0,
+events) and the CPU baseline (the reference's own FP64 solve_3d, built from its sources
+into oracle/_ref, on the host's cores; the numpy oracle where that build is absent; a
+bounded sample of the same workload).  Extra keys: poisson_sigma0 (the same problem at sigma = 0,
 FP64 and the fastest mixed variant), influence (the 4x4x2-subdomain
 influence-matrix assembly, responses/s of device time) and ns_steps_per_s
 (100 timed steps of the 192x256x192 channel, FP64 and fp16x2 velocity solves).
@@ -70,7 +71,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-solves", type=int, default=2)
+    ap.add_argument("--cpu-solves", type=int, default=0,
+                    help="CPU baseline calls (0: 20 of the reference build, 2 of the numpy port)")
     ap.add_argument("--workload", choices=("helmholtz", "influence", "md", "sgs"),
                     default="helmholtz")
     ap.add_argument("--sgs-fine", type=int, nargs=3, default=(256, 256, 256))
@@ -377,36 +379,57 @@ def ns_segment(args, cpu_steps=0):
 
 
 def cpu_baseline(args, gx, gy, gz, g, sigma, a64, amx):
-    """The FP64 oracle (numpy + OpenBLAS dgemm, the reference's algorithm)
-    timed on the host on a bounded sample: cpu_solves complete solves of the
-    same problem.  Also states the GPU results' distance to it."""
+    """The reference's own FP64 solver on the host: solve_3d
+    (class_solver_3d.f90:83) compiled from the reference's sources by
+    oracle/build_ref.sh (amdflang, MKL's LAPACK/BLAS on GNU OpenMP threads)
+    into oracle/_ref/, timed over a bounded sample of calls on the same
+    problem (the coefficient set-up is timed apart).  Without that build, the
+    FP64 oracle (numpy + OpenBLAS dgemm, the reference's algorithm) instead.
+    Also states the GPU results' distance to the baseline's solution."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import mf_oracle as o
+    import ref_solver
 
-    try:
-        from threadpoolctl import threadpool_info
-
-        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:  # noqa: BLE001
-        cores = os.cpu_count() or 1
     n = g.shape[0]
-    log("cpu baseline: oracle setup")
-    sc = o.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
-    z = [np.zeros((n - 2, n - 2, 2), order="F") for _ in range(3)]
-    u = np.zeros_like(g)
-    t0 = time.perf_counter()
-    for _ in range(args.cpu_solves):
-        o.solver_3d(sc, g, u, *z, sigma)
-    t = (time.perf_counter() - t0) / args.cpu_solves
-    log("cpu baseline: %.3f s/solve on %d threads" % (t, cores))
+    if ref_solver.available():
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        calls = args.cpu_solves or 20
+        log("cpu baseline: reference solve_3d, %d calls on %d threads" % (calls, threads))
+        u, t, t_setup = ref_solver.solve(gx, gy, gz, g, sigma, nrep=calls, threads=threads)
+        log("cpu baseline: %.3f s/solve (set-up %.2f s)" % (t, t_setup))
+        base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": threads,
+                "kind": "reference",
+                "sample": "%d solve_3d calls of the reference on the %d^3 problem (amdflang -O3, "
+                          "MKL dgemm on %d threads; solver_coeffs_init_3d %.2f s, not counted)"
+                          % (calls, n, threads, t_setup)}
+        key = "err_vs_reference"
+    else:
+        import mf_oracle as o
+
+        try:
+            from threadpoolctl import threadpool_info
+
+            cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+        except Exception:  # noqa: BLE001
+            cores = os.cpu_count() or 1
+        calls = args.cpu_solves or 2
+        log("cpu baseline: oracle setup")
+        sc = o.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6)
+        z = [np.zeros((n - 2, n - 2, 2), order="F") for _ in range(3)]
+        u = np.zeros_like(g)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            o.solver_3d(sc, g, u, *z, sigma)
+        t = (time.perf_counter() - t0) / calls
+        log("cpu baseline: %.3f s/solve on %d threads" % (t, cores))
+        base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+                "sample": "%d full %d^3 FP64 solves (numpy/OpenBLAS dgemm)" % (calls, n)}
+        key = "err_vs_oracle64"
     inner = (slice(1, -1),) * 3
     scale = float(np.abs(u[inner]).max())
-    errs = {"err_vs_oracle64": {"fp64_gpu": float(np.abs(a64[inner] - u[inner]).max() / scale),
-                                "mixed_gpu": float(np.abs(amx[inner] - u[inner]).max() / scale)}}
-    base = {"value": round(1.0 / t, 4), "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": "%d full %d^3 FP64 solves (numpy/OpenBLAS dgemm)" % (args.cpu_solves, n)}
+    errs = {key: {"fp64_gpu": float(np.abs(a64[inner] - u[inner]).max() / scale),
+                  "mixed_gpu": float(np.abs(amx[inner] - u[inner]).max() / scale)}}
     return base, errs
 
 

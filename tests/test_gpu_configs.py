@@ -61,6 +61,7 @@ def c2():
     n = g.shape[0]
     state = {
         "g": g,
+        "grids": (gx, gy, gz),
         "sc_o": o.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6),
         "sc": mf.SolverCoeffs3D(gx, gy, gz, (1,) * 6, 6),
         "gd": mf.to_device(g),
@@ -97,6 +98,27 @@ def test_c2_fp64_matches_oracle(c2, sigma):
           % (sigma, err, r_gpu, r_ora), flush=True)
     assert err <= 1e-12
     assert r_gpu <= 1e-12
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("sigma", C2_SIGMAS)
+def test_c2_fp64_matches_reference_build(c2, sigma):
+    """The GPU's FP64 solve against the reference's own solve_3d
+    (class_solver_3d.f90:83, compiled from its sources by oracle/build_ref.sh
+    with MKL's LAPACK/BLAS) on the same problem."""
+    import ref_solver
+    if not ref_solver.available():
+        pytest.skip("reference build absent (oracle/_ref: sh oracle/build_ref.sh)")
+    mf, torch = _mf()
+    print("run c2 fp64 vs reference build sigma=%g" % sigma, flush=True)
+    ref, t_ref, _ = ref_solver.solve(*c2["grids"], c2["g"], sigma)
+    u = mf.fortran_zeros(c2["g"].shape)
+    c2["sc"].solve(c2["gd"], u, c2["bc"], sigma, mf.MF_FP64)
+    inner = (slice(1, -1),) * 3
+    err = _rel(mf.to_host(u)[inner], ref[inner])
+    print("c2 fp64 sigma=%g: |gpu-reference| %.2e (reference solve_3d %.3f s)"
+          % (sigma, err, t_ref), flush=True)
+    assert err <= 1e-12
 
 
 @pytest.mark.timeout(600)
