@@ -1,12 +1,16 @@
-"""The reference's own FP64 solver, run as a program (test and baseline
-infrastructure only, never the product path).
+"""The reference's own FP64 solver and compact derivatives, run as programs
+(test and baseline infrastructure only, never the product path).
 
 `oracle/build_ref.sh` compiles the reference's solver sources
 (src/class_solver/class_solver_3d.f90, class_solver_coefficient_3d.f90,
 src/precision/precision.f90) with the driver `oracle/ref_solver_driver.f90`
-into `oracle/_ref/ref_solver_driver`; `solve` writes a problem for it, runs
-it and reads back the solution of solve_3d (class_solver_3d.f90:83) and its
-time per call.
+into `oracle/_ref/ref_solver_driver`, and its derivative sources
+(src/class_derivatives/class_derivatives.f90,
+class_derivatives_coefficient.f90) with `oracle/ref_der_driver.f90` into
+`oracle/_ref/ref_der_driver`.  `solve` writes a problem, runs the solver and
+reads back the solution of solve_3d (class_solver_3d.f90:83) and its time per
+call; `derivatives` returns der, dder, der_s and dder_s
+(class_derivatives.f90:186-260) of a set of lines.
 """
 from __future__ import annotations
 
@@ -19,6 +23,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "_ref", "ref_solver_driver")
+DER_EXE = os.path.join(HERE, "_ref", "ref_der_driver")
 BUILD = os.path.join(HERE, "build_ref.sh")
 REF = "/root/reference"
 
@@ -31,6 +36,10 @@ def _big_stack():
 
 def available() -> bool:
     return os.access(EXE, os.X_OK)
+
+
+def der_available() -> bool:
+    return os.access(DER_EXE, os.X_OK)
 
 
 def build() -> bool:
@@ -76,3 +85,26 @@ def solve(gx, gy, gz, g, sigma: float, bcx=None, bcy=None, bcz=None, bct=(1,) * 
     t_solve, t_setup = float(out[0]), float(out[1])
     u = np.asfortranarray(out[2:].reshape((nx, ny, nz), order="F"))
     return u, t_solve, t_setup
+
+
+def derivatives(grid, x, so: int = 6, solver: bool = True):
+    """The reference's der, dder (and, in solver mode, der_s, dder_s) of the
+    lines x (n, nlines), grid of n points.  Returns a dict of (n, nlines)
+    arrays; der_s and dder_s hold 0 at the first and last points."""
+    if not der_available():
+        raise RuntimeError("reference derivatives not built: run `sh oracle/build_ref.sh`")
+    grid = np.asarray(grid, dtype=np.float64)
+    n = len(grid)
+    x = np.asarray(x, dtype=np.float64).reshape(n, -1, order="F")
+    nl = x.shape[1]
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([n, so, 1 if solver else 0, nl], dtype=np.int32).tofile(f)
+            grid.tofile(f)
+            x.ravel(order="F").tofile(f)
+        subprocess.run([DER_EXE, fin, fout], check=True, preexec_fn=_big_stack)
+        out = np.fromfile(fout, dtype=np.float64)
+    names = ["der", "dder"] + (["der_s", "dder_s"] if solver else [])
+    return {k: out[i * n * nl:(i + 1) * n * nl].reshape((n, nl), order="F")
+            for i, k in enumerate(names)}

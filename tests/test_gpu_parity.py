@@ -131,6 +131,31 @@ def test_der_lines():
         _check("lines " + name, got, ex, ora)
 
 
+@pytest.mark.parametrize("n,so", [(64, 6), (192, 6), (33, 4)])
+def test_der_lines_match_reference_build(n, so):
+    """der, dder, der_s, dder_s on the GPU against the reference's own
+    (class_derivatives.f90:186-260, compiled from its sources by
+    oracle/build_ref.sh) on a tanh-stretched grid."""
+    import ref_solver
+    if not ref_solver.der_available():
+        pytest.skip("reference build absent (oracle/_ref: sh oracle/build_ref.sh)")
+    mf, torch = _mf()
+    print("run der lines vs reference build n=%d so=%d" % (n, so), flush=True)
+    g = o.grid_tanh(n, 0, 1, 2.0)
+    dc = mf.DerivativeCoefficients(g, solver=True, so=so)
+    x = np.asfortranarray(np.random.default_rng(n).standard_normal((n, 45)))
+    ref = ref_solver.derivatives(g, x, so=so, solver=True)
+    xd = mf.to_device(x)
+    for name in ("der", "dder", "der_s", "dder_s"):
+        out = mf.fortran_zeros((n, 45))
+        getattr(dc, name)(xd, out)
+        rows = slice(1, -1) if name.endswith("_s") else slice(None)
+        got, want = mf.to_host(out)[rows], ref[name][rows]
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print("  %s: |gpu-reference| %.2e" % (name, err), flush=True)
+        assert err <= 1e-12, (name, err)
+
+
 def test_derivative_rejects_bad_sizes():
     mf, torch = _mf()
     dc = mf.DerivativeCoefficients(o.grid_uniform(20), solver=False, so=6)

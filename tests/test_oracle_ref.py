@@ -1,9 +1,14 @@
-"""The numpy oracle's solver against the reference's own solve_3d
-(class_solver_3d.f90:83-297, with solver_coeffs_init_3d :380-466), compiled
-from the reference's sources by oracle/build_ref.sh (MKL's LAPACK/BLAS) into
-oracle/_ref/.  Dirichlet and mixed Dirichlet/Neumann boundaries with nonzero
-face values, a stretched grid, Helmholtz and Poisson sigma.  Skipped when the
-reference build is absent."""
+"""The numpy oracle against the reference's own code, compiled from the
+reference's sources by oracle/build_ref.sh (MKL's LAPACK/BLAS) into
+oracle/_ref/:
+
+* solve_3d (class_solver_3d.f90:83-297, with solver_coeffs_init_3d
+  :380-466): Dirichlet and mixed Dirichlet/Neumann boundaries with nonzero
+  face values, a stretched grid, Helmholtz and Poisson sigma;
+* der, dder, der_s, dder_s (class_derivatives.f90:111-260) on uniform and
+  tanh-stretched grids, orders so = 4, 6, 8.
+
+Skipped when the reference build is absent."""
 import numpy as np
 import pytest
 
@@ -38,3 +43,28 @@ def test_oracle_solver_matches_reference_build(shape, bct, sigma):
     # interior: the solve itself; the faces: the Neumann extrapolation too
     assert np.abs(u[inner] - ref[inner]).max() <= 1e-13 * scale
     assert np.abs(u - ref).max() <= 1e-12 * scale
+
+
+DER_CASES = [(64, "uniform", 6), (40, "tanh", 6), (33, "tanh", 4), (192, "tanh", 6),
+             (12, "tanh", 6), (50, "uniform", 8)]
+
+
+@pytest.mark.skipif(not ref_solver.der_available(),
+                    reason="reference build absent (oracle/_ref: sh oracle/build_ref.sh)")
+@pytest.mark.parametrize("n,grid,so", DER_CASES)
+def test_oracle_derivatives_match_reference_build(n, grid, so):
+    """Both sides solve the same Taylor systems for the scheme coefficients
+    with LAPACK in FP64 (numpy's OpenBLAS, MKL); at so = 8 those systems are
+    ill-conditioned enough that the two LAPACKs' coefficients differ near
+    1e-12 relative (the oracle's own are pinned against 40-digit arithmetic
+    in test_oracle.py), hence the wider bar there."""
+    g = o.grid_uniform(n, 0.0, 1.0) if grid == "uniform" else o.grid_tanh(n, 0.0, 1.0, 2.0)
+    x = np.random.default_rng(n).standard_normal((n, 9))
+    ref = ref_solver.derivatives(g, x, so=so, solver=True)
+    dc = o.DerCoeffs(g, True, so)
+    bar = 1e-12 if so <= 6 else 2e-11
+    for name in ("der", "dder", "der_s", "dder_s"):
+        rows = slice(1, -1) if name.endswith("_s") else slice(None)
+        got, want = getattr(dc, name)(x)[rows], ref[name][rows]
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= bar, (name, err)
