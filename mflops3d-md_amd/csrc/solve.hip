@@ -86,18 +86,22 @@ __global__ void __launch_bounds__(256)
 // so the modes the divide keeps are those of the reference in exact
 // arithmetic; in floating point the incompatible part would otherwise leak
 // into them through the rounding of their eigenvectors.
-// Pass 1: one block per z-plane, part[k] = w0z(k) sum_ij w0x(i) w0y(j) R(i,j,k).
+// Pass 1: NULL_SLICES blocks per z-plane, each over a contiguous run of
+// rows: part[k NULL_SLICES + s] = w0z(k) sum_(ij in slice s) w0x(i) w0y(j) R(i,j,k).
+constexpr int NULL_SLICES = 8;
 __global__ void __launch_bounds__(256)
     null_dot_kernel(const double *__restrict__ R, int n1, int n2, const double *__restrict__ w0,
                     int n3, double *__restrict__ part) {
-  const int k = blockIdx.x;
+  const int k = blockIdx.x, sl = blockIdx.y;
   const double *wx = w0, *wy = w0 + n1, *wz = w0 + n1 + n2;
-  const long n12 = (long)n1 * n2;
-  const double *Rk = R + n12 * k;
+  const int rows = (n2 + NULL_SLICES - 1) / NULL_SLICES;
+  const int j0 = sl * rows, j1 = min(n2, j0 + rows);
+  const double *Rk = R + (long)n1 * n2 * k + (long)n1 * j0;
+  const int np = j1 > j0 ? (j1 - j0) * n1 : 0;
   double acc = 0.0;
-  for (long p = threadIdx.x; p < n12; p += 256) {
-    const int i = (int)(p % n1), j = (int)(p / n1);
-    acc += wx[i] * wy[j] * Rk[p];
+  for (int p = threadIdx.x; p < np; p += 256) {
+    const int jj = p / n1, i = p - jj * n1;
+    acc += wx[i] * wy[j0 + jj] * Rk[p];
   }
   __shared__ double red[256];
   red[threadIdx.x] = acc;
@@ -106,22 +110,24 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[k] = red[0] * wz[k];
+  if (threadIdx.x == 0) part[k * NULL_SLICES + sl] = red[0] * wz[k];
 }
 
-// Pass 2: every block sums the plane partials in the same order (so all see
+// Pass 2: every block sums the npart partials in the same order (so all see
 // the same c0) and subtracts c0 from its points.
 __global__ void __launch_bounds__(256)
-    null_sub_kernel(double *__restrict__ R, long N, const double *__restrict__ part, int n3,
+    null_sub_kernel(double *__restrict__ R, long N, const double *__restrict__ part, int npart,
                     double inv_wsum) {
-  __shared__ double c0s;
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int k = 0; k < n3; ++k) t += part[k];
-    c0s = t * inv_wsum;
-  }
+  __shared__ double red[256];
+  double t = 0.0;
+  for (int q = threadIdx.x; q < npart; q += 256) t += part[q];
+  red[threadIdx.x] = t;
   __syncthreads();
-  const double c0 = c0s;
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double c0 = red[0] * inv_wsum;
   for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long)gridDim.x * 256) R[p] -= c0;
 }
 
@@ -612,7 +618,7 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->Vhp2 = (_Float16 *)take((2 * (size_t)h->wrows[2] * h->kp[2] + 3) / 4);
     h->fmx = (unsigned *)take(4);
     h->w0 = take((size_t)n[0] + n[1] + n[2]);
-    h->npart = take(n[2]);
+    h->npart = take((size_t)n[2] * mf::NULL_SLICES);
     return off;
   };
   const size_t nd = carve(nullptr);
@@ -1264,11 +1270,12 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
     }
     if (h->null_mode && sigma == 0.0) {
       ProfScope ps(h, P_FOLD, 16.0 * N, st);
-      hipLaunchKernelGGL(mf::null_dot_kernel, dim3(h->n3), dim3(256), 0, st, (const double *)h->T2,
-                         h->n1, h->n2, (const double *)h->w0, h->n3, h->npart);
+      hipLaunchKernelGGL(mf::null_dot_kernel, dim3(h->n3, mf::NULL_SLICES), dim3(256), 0, st,
+                         (const double *)h->T2, h->n1, h->n2, (const double *)h->w0, h->n3,
+                         h->npart);
       const long nb = std::min<long>((N + 255) / 256, 4096);
       hipLaunchKernelGGL(mf::null_sub_kernel, dim3((unsigned)nb), dim3(256), 0, st, h->T2, N,
-                         (const double *)h->npart, h->n3, h->inv_wsum);
+                         (const double *)h->npart, h->n3 * mf::NULL_SLICES, h->inv_wsum);
     }
     transforms<double, double>(h, h->T2, h->T1, h->T2, h->Wx, h->Wyt, h->Wzt, h->Vx, h->Vyt,
                                h->Vzt, sigma, u, false, st);
