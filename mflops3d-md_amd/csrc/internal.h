@@ -7,6 +7,7 @@
 
 struct mf_der;
 struct mf_solver3d;
+struct mf_solve_report;
 
 namespace mf {
 
@@ -69,6 +70,17 @@ int influence_assemble(const std::vector<InflSub> &subs, double sigma, bool grap
 // host copies of a derivative handle's tables (order 1|2, sub 0|1)
 bool der_host_tables(const struct ::mf_der *dc, int order, int sub, const double **dl_lu,
                      const double **dr, int *n);
+
+// mf_solver3d_solve in two halves, for callers that overlap solves of
+// different handles on different streams.  _begin enqueues the solve on the
+// stream (FP64: all of it; mixed: the right-hand side, the speculative
+// sweeps, u's faces and the copy of the residual norms to the host) and does
+// not wait; _end waits for that stream and finishes the refinement (further
+// sweeps if the tolerance is not met), reporting as mf_solver3d_solve.
+int solver3d_solve_begin(struct ::mf_solver3d *h, const double *g, double sigma, double *u,
+                         const double *bcx, const double *bcy, const double *bcz, int precision,
+                         double tol, int max_iter, void *stream);
+int solver3d_solve_end(struct ::mf_solver3d *h, struct ::mf_solve_report *rep);
 
 // Squared spacings of one grid for the eddy viscosity's filter width
 // (one-sided on the first and last points, centred inside).

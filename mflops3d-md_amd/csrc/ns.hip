@@ -564,6 +564,11 @@ struct mf_ns {
   int tou = 2, top = 2, nlt = 2;
   mf_der *dc[3] = {nullptr, nullptr, nullptr};
   mf_solver3d *scu = nullptr, *scp = nullptr;
+  // the v and w velocity solves on handles and streams of their own, so that
+  // the three solves overlap (MF_NS_CONCURRENT=0: one handle, in turn)
+  mf_solver3d *scu_vw[2] = {nullptr, nullptr};
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   void *mem = nullptr;
   double *q[3][NT], *p[NT], *phi[NT], *nl[3][NT], *dphi[NT][3];
   bool nl_ok[NT], dphi_ok[NT];
@@ -782,7 +787,8 @@ int velocity_solves(mf_ns *s, const Extrap &ev, bool rhs_in, double *const dp[3]
   const int *it = s->it;
   const int n1 = it[0], L0 = it[NT - 1], L1 = it[NT - 2], L2 = it[NT - 3];
   const long sx = 2L * (s->ny - 2) * (s->nz - 2), sy = 2L * (s->nx - 2) * (s->nz - 2);
-  for (int c = 0; c < 3; ++c) {
+  const double sigma = -s->re * s->fac[0];
+  auto rhs = [&](int c, double *out) {
     RhsArgs r;
     std::memset(&r, 0, sizeof(r));
     r.q0 = s->q[c][L0]; r.q1 = s->q[c][L1]; r.q2 = s->q[c][L2];
@@ -798,13 +804,36 @@ int velocity_solves(mf_ns *s, const Extrap &ev, bool rhs_in, double *const dp[3]
     }
     r.dp = dp ? dp[c] : nullptr;
     r.re = s->re;
-    r.out = s->T[0];
+    r.out = out;
     r.N = s->N;
     r.d = s->d;
     hipLaunchKernelGGL(rhs_kernel, dim3(nblk(s->N)), dim3(256), 0, st, r);
+  };
+  if (!s->scu_vw[0]) {
+    for (int c = 0; c < 3; ++c) {
+      rhs(c, s->T[0]);
+      NSCHK(hipGetLastError());
+      RC(mf_solver3d_solve(s->scu, s->T[0], sigma, s->q[c][n1], bc[c], bc[c] + sx,
+                           bc[c] + sx + sy, precision, tol, 0, nullptr, st));
+    }
+  } else {
+    // right-hand sides into T[0..2] on st; u's solve on st, v's and w's on
+    // the side streams after them; the host waits for all three (mixed
+    // precision checks each residual), and st waits for the side streams
+    for (int c = 0; c < 3; ++c) rhs(c, s->T[c]);
     NSCHK(hipGetLastError());
-    RC(mf_solver3d_solve(s->scu, s->T[0], -s->re * s->fac[0], s->q[c][n1], bc[c], bc[c] + sx,
-                         bc[c] + sx + sy, precision, tol, 0, nullptr, st));
+    NSCHK(hipEventRecord(s->ev_fork, st));
+    for (int k = 0; k < 2; ++k) NSCHK(hipStreamWaitEvent(s->side[k], s->ev_fork, 0));
+    mf_solver3d *hs[3] = {s->scu, s->scu_vw[0], s->scu_vw[1]};
+    hipStream_t ss[3] = {st, s->side[0], s->side[1]};
+    for (int c = 0; c < 3; ++c)
+      RC(mf::solver3d_solve_begin(hs[c], s->T[c], sigma, s->q[c][n1], bc[c], bc[c] + sx,
+                                  bc[c] + sx + sy, precision, tol, 0, ss[c]));
+    for (int c = 0; c < 3; ++c) RC(mf::solver3d_solve_end(hs[c], nullptr));
+    for (int k = 0; k < 2; ++k) {
+      NSCHK(hipEventRecord(s->ev_join[k], s->side[k]));
+      NSCHK(hipStreamWaitEvent(st, s->ev_join[k], 0));
+    }
   }
   s->nl_ok[n1] = false;
   return MF_OK;
@@ -1024,6 +1053,12 @@ extern "C" int mf_ns_destroy(mf_ns *s) {
   for (int a = 0; a < 3; ++a) mf_der_destroy(s->dc[a]);
   mf_solver3d_destroy(s->scu);
   mf_solver3d_destroy(s->scp);
+  for (int k = 0; k < 2; ++k) {
+    mf_solver3d_destroy(s->scu_vw[k]);
+    if (s->side[k]) (void)hipStreamDestroy(s->side[k]);
+    if (s->ev_join[k]) (void)hipEventDestroy(s->ev_join[k]);
+  }
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
   if (s->mem) (void)hipFree(s->mem);
   if (s->lesmem) (void)hipFree(s->lesmem);
   if (s->sgsmem) (void)hipFree(s->sgsmem);
@@ -1061,6 +1096,23 @@ extern "C" int mf_ns_create(const double *gx, int nx, const double *gy, int ny, 
   if (rc == MF_OK) rc = mf_der_create(gz, nz, 1, so, &s->dc[2]);
   if (rc == MF_OK) rc = mf_solver3d_create(gx, nx, gy, ny, gz, nz, bu, so, &s->scu);
   if (rc == MF_OK) rc = mf_solver3d_create(gx, nx, gy, ny, gz, nz, bp, so, &s->scp);
+  bool conc = true;
+  if (const char *env = std::getenv("MF_NS_CONCURRENT")) conc = std::atoi(env) != 0;
+  for (int k = 0; conc && k < 2 && rc == MF_OK; ++k)
+    rc = mf_solver3d_create(gx, nx, gy, ny, gz, nz, bu, so, &s->scu_vw[k]);
+  if (conc && rc == MF_OK) {
+    // default priority, non-blocking: no implicit ordering with the null stream
+    hipError_t he = hipSuccess;
+    for (int k = 0; k < 2 && he == hipSuccess; ++k) {
+      he = hipStreamCreateWithFlags(&s->side[k], hipStreamNonBlocking);
+      if (he == hipSuccess) he = hipEventCreateWithFlags(&s->ev_join[k], hipEventDisableTiming);
+    }
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming);
+    if (he != hipSuccess) {
+      mf_ns_destroy(s);
+      return mf::fail(MF_ERR_DEVICE, std::string("mf_ns_create: streams: ") + hipGetErrorString(he));
+    }
+  }
   if (rc != MF_OK) {
     const std::string msg = mf_last_error();
     mf_ns_destroy(s);

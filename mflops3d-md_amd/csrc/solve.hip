@@ -240,6 +240,16 @@ struct mf_solver3d {
   double *zq = nullptr;                 // Mz, Rz rows on planes b(k)..b(k)+4, e_z1, e_z2 (n3, 12)
   int res_cz = 0;                       // output planes per z-streaming residual block; 0: plane pipeline
   double *hnorms = nullptr;             // pinned host mirror
+  // a mixed solve enqueued by solver3d_solve_begin and not yet finished by
+  // solver3d_solve_end: its arguments and the sweeps enqueued so far
+  struct Pending {
+    bool active = false;
+    const double *g = nullptr, *bcx = nullptr, *bcy = nullptr, *bcz = nullptr;
+    double *u = nullptr;
+    double sigma = 0.0, tol = 0.0;
+    int precision = 0, max_iter = 0, it = 0;
+    hipStream_t st = nullptr;
+  } pending;
   // singular problems (sigma = 0, a zero eigenvalue on every axis): the
   // null rows w0x | w0y | w0z of inv(V), 1 / (sum w0x sum w0y sum w0z), and
   // n3 plane partials
@@ -1247,10 +1257,36 @@ int rhs_prime(mf_solver3d *h, const double *g, const double *bcx, const double *
 }
 }  // namespace
 
-extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, double *u,
-                                 const double *bcx, const double *bcy, const double *bcz,
-                                 int precision, double tol, int max_iter, mf_solve_report *rep,
-                                 void *stream) {
+namespace {
+// One inner (low-precision) solve of the mixed path on r' (R32): u = S(r')
+// or u += S(r').
+void inner_solve(mf_solver3d *h, int precision, double sigma, double *u, bool accumulate,
+                 hipStream_t st) {
+  if (precision == MF_MIXED_BF16X3)
+    transforms_bf3(h, h->R32, h->F1, h->F2, sigma, u, accumulate, st);
+  else if (precision == MF_MIXED_FP16X2)
+    transforms_h2(h, h->R32, h->F1, h->F2, sigma, u, accumulate, st);
+  else
+    transforms<float, float>(h, h->R32, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
+                             h->Vyt32, h->Vzt32, sigma, u, accumulate, st);
+}
+
+// u's faces from the latest sweep's interior and the norms of its residuals
+// to the host mirror (pinned), both enqueued: the host reads the mirror only
+// after a wait on the stream
+int enqueue_check(mf_solver3d *h) {
+  auto &p = h->pending;
+  int rc = launch_faces(h, p.u, p.bcx, p.bcy, p.bcz, p.st);
+  if (rc != MF_OK) return rc;
+  HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, (p.it + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                        p.st));
+  return MF_OK;
+}
+}  // namespace
+
+int mf::solver3d_solve_begin(mf_solver3d *h, const double *g, double sigma, double *u,
+                             const double *bcx, const double *bcy, const double *bcz,
+                             int precision, double tol, int max_iter, void *stream) {
   if (!h || !g || !u || !bcx || !bcy || !bcz)
     return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: null pointer");
   if (precision != MF_FP64 && precision != MF_MIXED && precision != MF_MIXED_BF16X3 &&
@@ -1261,7 +1297,7 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   if (!std::isfinite(sigma)) return mf::fail(MF_ERR_ARG, "mf_solver3d_solve: sigma is not finite");
   hipStream_t st = (hipStream_t)stream;
   const long N = (long)h->n1 * h->n2 * h->n3;
-  if (rep) { rep->iterations = 0; rep->residual = 0.0; rep->residual0 = 0.0; }
+  h->pending.active = false;
   if (precision == MF_FP64) {
     {
       ProfScope ps(h, P_FOLD, 16.0 * N, st);
@@ -1285,21 +1321,12 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   // ---- mixed precision: FP32-class MFMA transforms, FP64 residual refinement
   if (!(tol > 0.0)) tol = 1e-12;
   if (max_iter <= 0) max_iter = 8;
-  auto inner = [&](const float *in, bool accumulate) {
-    if (precision == MF_MIXED_BF16X3)
-      transforms_bf3(h, in, h->F1, h->F2, sigma, u, accumulate, st);
-    else if (precision == MF_MIXED_FP16X2)
-      transforms_h2(h, in, h->F1, h->F2, sigma, u, accumulate, st);
-    else
-      transforms<float, float>(h, in, h->F1, h->F2, h->Wx32, h->Wyt32, h->Wzt32, h->Vx32,
-                               h->Vyt32, h->Vzt32, sigma, u, accumulate, st);
-  };
   if (max_iter > MF_MAX_REFINE) max_iter = MF_MAX_REFINE;
   const bool scales = precision == MF_MIXED_FP16X2;
   int rc = rhs_prime(h, g, bcx, bcy, bcz, h->norms + 0, scales, st);
   if (rc != MF_OK) return rc;
   // first low-precision solve: u_int = S(b'); norms[it] = ||r'||^2 after it solves
-  inner(h->R32, false);
+  inner_solve(h, precision, sigma, u, false, st);
   rc = residual(h, u, g, sigma, h->norms + 1, scales, st);
   if (rc != MF_OK) return rc;
   int it = 1;
@@ -1307,37 +1334,62 @@ extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, 
   // tolerance the first correction is enqueued without waiting for the
   // first residual (one host round trip less).
   if (tol < 1e-6 && max_iter >= 2) {
-    inner(h->R32, true);
+    inner_solve(h, precision, sigma, u, true, st);
     rc = residual(h, u, g, sigma, h->norms + 2, scales, st);
     if (rc != MF_OK) return rc;
     it = 2;
   }
+  auto &p = h->pending;
+  p.g = g; p.u = u; p.bcx = bcx; p.bcy = bcy; p.bcz = bcz;
+  p.sigma = sigma; p.tol = tol; p.precision = precision; p.max_iter = max_iter; p.it = it;
+  p.st = st;
+  rc = enqueue_check(h);
+  if (rc != MF_OK) return rc;
+  p.active = true;
+  return MF_OK;
+}
+
+int mf::solver3d_solve_end(mf_solver3d *h, mf_solve_report *rep) {
+  if (rep) { rep->iterations = 0; rep->residual = 0.0; rep->residual0 = 0.0; }
+  if (!h || !h->pending.active) return MF_OK;  // FP64: nothing left to do
+  auto &p = h->pending;
+  p.active = false;
+  const bool scales = p.precision == MF_MIXED_FP16X2;
   double rel = 0.0;
   for (;;) {
-    // u's faces from this sweep's interior, enqueued before the host reads the
-    // norms (no launch after the wait); a further sweep writes them again
-    rc = launch_faces(h, u, bcx, bcy, bcz, st);
-    if (rc != MF_OK) return rc;
-    HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, (it + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamSynchronize(p.st));
     const double b2 = h->hnorms[0];
     auto relres = [&](double r2) { return (b2 > 0.0) ? std::sqrt(r2 / b2) : std::sqrt(r2); };
-    rel = relres(h->hnorms[it]);
+    rel = relres(h->hnorms[p.it]);
     if (rep) {
       rep->residual0 = relres(h->hnorms[1]);
       rep->residual = rel;
-      rep->iterations = it;
+      rep->iterations = p.it;
     }
-    if (!(rel > tol) || it >= max_iter) break;
-    inner(h->R32, true);
-    ++it;
-    rc = residual(h, u, g, sigma, h->norms + it, scales, st);
+    if (!(rel > p.tol) || p.it >= p.max_iter) break;
+    inner_solve(h, p.precision, p.sigma, p.u, true, p.st);
+    ++p.it;
+    int rc = residual(h, p.u, p.g, p.sigma, h->norms + p.it, scales, p.st);
+    if (rc != MF_OK) return rc;
+    // a further sweep writes u's faces again
+    rc = enqueue_check(h);
     if (rc != MF_OK) return rc;
   }
-  if (!(rel <= tol))
+  if (!(rel <= p.tol))
     return mf::fail(MF_ERR_NOCONV, "mf_solver3d_solve: refinement stopped at relative residual " +
                                        std::to_string(rel));
   return MF_OK;
+}
+
+extern "C" int mf_solver3d_solve(mf_solver3d *h, const double *g, double sigma, double *u,
+                                 const double *bcx, const double *bcy, const double *bcz,
+                                 int precision, double tol, int max_iter, mf_solve_report *rep,
+                                 void *stream) {
+  if (rep) { rep->iterations = 0; rep->residual = 0.0; rep->residual0 = 0.0; }
+  const int rc = mf::solver3d_solve_begin(h, g, sigma, u, bcx, bcy, bcz, precision, tol,
+                                          max_iter, stream);
+  if (rc != MF_OK) return rc;
+  return mf::solver3d_solve_end(h, rep);
 }
 
 extern "C" int mf_solver3d_solve_host(mf_solver3d *h, const double *g, double sigma, double *u,

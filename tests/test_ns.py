@@ -380,6 +380,28 @@ def test_ns_advective_form_matches_oracle():
     _compare(mf, dev, orc, 1e-10, "nlt=1")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [0, 3], ids=["fp64", "fp16x2"])
+def test_ns_concurrent_velocity_solves_match_sequential(monkeypatch, prec):
+    """The three velocity solves on three handles and streams (the default)
+    against one handle in turn (MF_NS_CONCURRENT=0): the same kernels on the
+    same data, so the fields agree bit for bit."""
+    import mflops as mf
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((40, 48, 36))
+    re, dt, steps = 3000.0, 1e-3, 3
+    runs = {}
+    for conc in ("1", "0"):
+        print("run ns concurrent=%s precision=%d" % (conc, prec), flush=True)
+        monkeypatch.setenv("MF_NS_CONCURRENT", conc)
+        dev = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt)
+        dev.set_flow_rate(True, 2.0 / re)
+        dev.step(steps, prec, tol=1e-13)
+        runs[conc] = [mf.to_host(t) for t in dev.get()]
+    for name, a, b in zip(("u", "v", "w", "p", "phi"), runs["1"], runs["0"]):
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+
+
 def _orc_fields(orc):
     return list(orc.velocity()) + [orc.pressure(), orc.phi_latest()]
 
