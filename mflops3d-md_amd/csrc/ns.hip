@@ -626,6 +626,26 @@ namespace {
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
+// Three independent launches launch(k, stream), k = 0, 1, 2 (one per axis or
+// component): k = 0 on st, 1 and 2 on the side streams when they exist, with
+// a fork and a join by events; otherwise all three on st in turn.
+template <class F>
+int fork3(mf_ns *s, hipStream_t st, F &&launch) {
+  if (!s->side[0]) {
+    for (int k = 0; k < 3; ++k) RC(launch(k, st));
+    return MF_OK;
+  }
+  NSCHK(hipEventRecord(s->ev_fork, st));
+  for (int k = 0; k < 2; ++k) NSCHK(hipStreamWaitEvent(s->side[k], s->ev_fork, 0));
+  hipStream_t ss[3] = {st, s->side[0], s->side[1]};
+  for (int k = 0; k < 3; ++k) RC(launch(k, ss[k]));
+  for (int k = 0; k < 2; ++k) {
+    NSCHK(hipEventRecord(s->ev_join[k], s->side[k]));
+    NSCHK(hipStreamWaitEvent(st, s->ev_join[k], 0));
+  }
+  return MF_OK;
+}
+
 // extrapolation coefficients on levels it(nt), it(nt-1), it(nt-2)
 void extrap_coefs(int order, double c[3]) {
   c[0] = c[1] = c[2] = 0.0;
@@ -665,16 +685,18 @@ int nonlinear(mf_ns *s, int L, hipStream_t st) {
                        (const double *)v, (const double *)w, uu, uv, uw, vv, vw, ww, N);
     // axis a: d_a of (u, v, w) and of (q u_a), q = u, v, w
     const double *prod[3][3] = {{uu, uv, uw}, {uv, vv, vw}, {uw, vw, ww}};
-    for (int a = 0; a < 3; ++a) {
+    RC(fork3(s, st, [&](int a, hipStream_t q) {
       const double *x[6] = {u, v, w, prod[a][0], prod[a][1], prod[a][2]};
       double *o[6] = {T[3 * a], T[3 * a + 1], T[3 * a + 2], T[15 + 3 * a], T[16 + 3 * a],
                       T[17 + 3 * a]};
-      RC(mf_der3d_batch(s->dc[a], a, 1, 6, x, o, s->nx, s->ny, s->nz, st));
-    }
+      return mf_der3d_batch(s->dc[a], a, 1, 6, x, o, s->nx, s->ny, s->nz, q);
+    }));
     for (int a = 0; a < 3; ++a)
       for (int c = 0; c < 3; ++c) n.e[a][c] = T[15 + 3 * a + c];
   } else {
-    for (int a = 0; a < 3; ++a) RC(der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
+    RC(fork3(s, st, [&](int a, hipStream_t q) {
+      return der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], q);
+    }));
   }
   s->grad_level = L;
   for (int c = 0; c < 3; ++c) n.out[c] = s->nl[c][L];
@@ -693,8 +715,9 @@ int les_force(mf_ns *s, int L, hipStream_t st) {
   double **T = s->T;
   const long N = s->N;
   if (s->grad_level != L) {
-    for (int a = 0; a < 3; ++a)
-      RC(der3(s, a, s->q[0][L], s->q[1][L], s->q[2][L], T[3 * a], T[3 * a + 1], T[3 * a + 2], st));
+    RC(fork3(s, st, [&](int a, hipStream_t q) {
+      return der3(s, a, s->q[0][L], s->q[1][L], s->q[2][L], T[3 * a], T[3 * a + 1], T[3 * a + 2], q);
+    }));
     s->grad_level = L;
   }
   const double *du[9];  // du[3 i + j] = d u_i / d x_j = T[3 j + i]
@@ -934,7 +957,7 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
     double *dp[3] = {T[21], T[22], T[23]};
     if (pt == 2) {
       launch_extrap(s, s->p, ep, T[20], st);
-      for (int a = 0; a < 3; ++a) RC(der(s, a, T[20], dp[a], st));
+      RC(fork3(s, st, [&](int a, hipStream_t q) { return der(s, a, T[20], dp[a], q); }));
     }
     RC(velocity_solves(s, ev, false, pt == 2 ? dp : nullptr, bc, precision, tol, st));
   }
@@ -953,7 +976,7 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
     r.d = s->d;
     hipLaunchKernelGGL(rp_kernel, dim3(nblk(N)), dim3(256), 0, st, r);
   }
-  for (int a = 0; a < 3; ++a) RC(der(s, a, T[a], T[3 + a], st));
+  RC(fork3(s, st, [&](int a, hipStream_t q) { return der(s, a, T[a], T[3 + a], q); }));
   hipLaunchKernelGGL(sum3_kernel, dim3(nblk(N)), dim3(256), 0, st, (const double *)T[3],
                      (const double *)T[4], (const double *)T[5], T[6], N);
   hipLaunchKernelGGL(bcphi_kernel, dim3(nblk(s->nbc)), dim3(256), 0, st, (const double *)T[0],
@@ -982,7 +1005,7 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
   // projection
   const long ne = 4L * (s->nx + s->ny + s->nz);
   hipLaunchKernelGGL(zero_edges_kernel, dim3(nblk(ne)), dim3(256), 0, st, s->phi[n1], s->d);
-  for (int a = 0; a < 3; ++a) RC(der(s, a, s->phi[n1], s->dphi[n1][a], st));
+  RC(fork3(s, st, [&](int a, hipStream_t q) { return der(s, a, s->phi[n1], s->dphi[n1][a], q); }));
   s->dphi_ok[n1] = true;
   {
     ProjArgs pj;
@@ -1017,7 +1040,7 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
   // velocity correction (pt >= 3) with the gradient of the new pressure
   if (pt >= 3) {
     double *dp[3] = {T[21], T[22], T[23]};
-    for (int a = 0; a < 3; ++a) RC(der(s, a, s->p[n1], dp[a], st));
+    RC(fork3(s, st, [&](int a, hipStream_t q) { return der(s, a, s->p[n1], dp[a], q); }));
     RC(velocity_solves(s, ev, true, dp, bc, precision, tol, st));
   }
   *done = false;
