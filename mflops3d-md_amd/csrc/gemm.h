@@ -57,6 +57,9 @@ struct EpiArgs {
   int nx, ny;
 };
 
+// Block tile BM x BN on 4 waves.  A side of 64/128/256 is split into 64-wide
+// wave tiles (4 x 4 MFMA blocks of 16); a side of 192 into 4 wave tiles of 48
+// (3 MFMA blocks), for eigenvector dimensions of 129..192 points.
 template <typename T, typename TA, typename TB, typename TC, int BM, int BN, int EPI>
 __global__ void __launch_bounds__(256)
     gemm_mode_kernel(const TA *__restrict__ A, long lda, long sA, const TB *__restrict__ B,
@@ -65,7 +68,10 @@ __global__ void __launch_bounds__(256)
   constexpr int BK = 16;
   constexpr int NA = BM * BK / 256;
   constexpr int NB = BN * BK / 256;
-  static_assert(BM * BN == 256 * 64, "4 waves of 64x64");
+  // wave grid WM x WN, each wave MI x NI MFMA blocks of 16 x 16
+  constexpr int MI = BM == 192 ? 3 : 4, NI = BN == 192 ? 3 : 4;
+  constexpr int WM = BM / (16 * MI), WN = BN / (16 * NI);
+  static_assert(WM * WN == 4 && WM * 16 * MI == BM && WN * 16 * NI == BN, "4 wave tiles");
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[BK][BM];
   __shared__ T Bs[BN][BK + 1];
@@ -77,16 +83,15 @@ __global__ void __launch_bounds__(256)
   const long n0 = (long)blockIdx.y * BN;
   A += (long)blockIdx.z * sA;
   if (EPI == EPI_STORE || EPI == EPI_DIVIDE) C += (long)blockIdx.z * sC;
-  // four 64x64 wave tiles arranged (BM/64) x (BN/64)
-  constexpr int WM = BM / 64;
-  const int wm = (wave % WM) * 64;
-  const int wn = (wave / WM) * 64;
+  // four wave tiles arranged WM x WN
+  const int wm = (wave % WM) * 16 * MI;
+  const int wn = (wave / WM) * 16 * NI;
 
-  acc_t acc[4][4];
+  acc_t acc[MI][NI];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < MI; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
+    for (int b = 0; b < NI; ++b) acc[a][b] = (acc_t){0, 0, 0, 0};
 
   T ra[NA], rb[NB];
   auto load = [&](int kt) {
@@ -122,16 +127,15 @@ __global__ void __launch_bounds__(256)
     if (kt + BK < K) load(kt + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T af[4], bf[4];
+      T af[MI], bf[NI];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        af[i] = As[kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
-        bf[i] = Bs[wn + i * 16 + (lane & 15)][kk + (lane >> 4)];
-      }
+      for (int i = 0; i < MI; ++i) af[i] = As[kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
+      for (int i = 0; i < NI; ++i) bf[i] = Bs[wn + i * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Mfma<T>::op(bf[ni], af[mi], acc[mi][ni]);
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = Mfma<T>::op(bf[ni], af[mi], acc[mi][ni]);
     }
   }
   // epilogue: lane holds C(m0+wm+16mi+lane%16, n0+wn+16ni+row(lane,r)).
@@ -144,7 +148,7 @@ __global__ void __launch_bounds__(256)
     i2 = mb / ep.n1;
   }
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi) {
+  for (int mi = 0; mi < MI; ++mi) {
     const long m = m0 + wm + mi * 16 + (lane & 15);
     if (mi > 0 && EPI == EPI_DIVIDE) {
       i1 += 16;
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(256)
       kb = nb / ep.n2;
     }
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
+    for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int dn = ni * 16 + Mfma<T>::row(lane, r);

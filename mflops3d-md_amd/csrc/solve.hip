@@ -775,13 +775,22 @@ void launch_tile(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, lo
 
 // Block tile by the eigenvector dimension n (M for the x-transform, whose
 // eigenvector matrix is the left operand; N for y/z): the tile side along n
-// is the smallest of 64/128/256 covering it, so every field element is read
-// once and few MFMA rows are padding.  WLEFT selects the orientation.
+// is the smallest of 64/128/192/256 covering it, so every field element is
+// read once and few MFMA rows are padding.  WLEFT selects the orientation.
+inline int gemm_mode_tile(int n) { return n > 128 && n <= 192 ? 192 : gemm_tile(n); }
+
 template <typename T, typename TA, typename TB, typename TC, bool WLEFT, int EPI>
 void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, long ldb, TC *C,
                  long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
                  hipStream_t st) {
-  const int t = gemm_tile(WLEFT ? M : N);
+  const int t = gemm_mode_tile(WLEFT ? M : N);
+  if (t == 192) {
+    if (WLEFT)
+      launch_tile<T, TA, TB, TC, 192, 64, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    else
+      launch_tile<T, TA, TB, TC, 64, 192, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    return;
+  }
   if (WLEFT) {
     if (t == 64)
       launch_tile<T, TA, TB, TC, 64, 256, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
