@@ -177,19 +177,25 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// out[0] = sum of n partial sums (one block of 1024 threads, fixed order).
-__global__ void __launch_bounds__(1024) sum_kernel(const double *__restrict__ part, long n,
-                                                   double *__restrict__ out) {
+// Several norms at once, one block of 1024 threads each, in a fixed order:
+// block i sums the n partials at part + i stride into *out.p[i].
+constexpr int NORM_SLOTS = 3;
+struct NormOuts {
+  double *p[NORM_SLOTS];
+};
+__global__ void __launch_bounds__(1024) sum_slots_kernel(const double *__restrict__ part,
+                                                         long stride, long n, NormOuts out) {
   __shared__ double s[1024];
+  const double *pp = part + stride * blockIdx.x;
   double a = 0.0;
-  for (long q = threadIdx.x; q < n; q += 1024) a += part[q];
+  for (long q = threadIdx.x; q < n; q += 1024) a += pp[q];
   s[threadIdx.x] = a;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = s[0];
+  if (threadIdx.x == 0) out.p[blockIdx.x][0] = s[0];
 }
 
 }  // namespace mf
@@ -236,7 +242,12 @@ struct mf_solver3d {
   double *ebd = nullptr;                // ex1 ex2 ey1 ey2 ez1 ez2: P-folded boundary rows
   double *fbf = nullptr;                // face products fbx fby fbz (mixed)
   double *norms = nullptr;              // device scalars
-  double *part = nullptr;               // per-block partial sums of norms
+  double *part = nullptr;               // per-block partial sums of norms, NORM_SLOTS arrays
+  size_t part_stride = 0;               // doubles per partials array
+  // residual norms whose partials are written but not yet summed (slots 0..)
+  int npend = 0;
+  unsigned pend_nb = 0;
+  double *pend_out[3] = {nullptr, nullptr, nullptr};
   double *zq = nullptr;                 // Mz, Rz rows on planes b(k)..b(k)+4, e_z1, e_z2 (n3, 12)
   int res_cz = 0;                       // output planes per z-streaming residual block; 0: plane pipeline
   double *hnorms = nullptr;             // pinned host mirror
@@ -602,7 +613,8 @@ extern "C" int mf_solver3d_create(const double *gx, int nx, const double *gy, in
     h->neum = take(neum.size());
     h->bandw = take(bandw.size());
     h->norms = take(MF_MAX_REFINE + 1);
-    h->part = take(nparts);
+    h->part = take(mf::NORM_SLOTS * nparts);
+    h->part_stride = nparts;
     h->T1 = take(N); h->T2 = take(N);
     h->F1 = (float *)take((N + 1) / 2);
     h->F2 = (float *)take((N + 1) / 2);
@@ -1204,19 +1216,38 @@ mf::ResArgs res_args(mf_solver3d *h, const double *u, const double *g, double si
   return a;
 }
 
+// The norms of the residual launches since the last flush, summed by one
+// launch (block i: the partials of pending norm i).
+int flush_norms(mf_solver3d *h, hipStream_t st) {
+  if (h->npend == 0) return MF_OK;
+  mf::NormOuts o;
+  for (int i = 0; i < mf::NORM_SLOTS; ++i) o.p[i] = h->pend_out[i];
+  hipLaunchKernelGGL(mf::sum_slots_kernel, dim3(h->npend), dim3(1024), 0, st,
+                     (const double *)h->part, (long)h->part_stride, (long)h->pend_nb, o);
+  h->npend = 0;
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+// One residual (or right-hand side) launch; its partial sums go to the next
+// free slot and are summed into norm2_dev at the next flush_norms (at once
+// when the slots are full), so that a solve's first norms take one launch.
 template <bool RES>
 int launch_residual(mf_solver3d *h, const mf::ResArgs &a, double *norm2_dev, hipStream_t st) {
   const bool zs = h->res_cz > 0;
   const int cz = zs ? h->res_cz : mf::RZ_CZ, ty = zs ? mf::ZS_TY : mf::RZ_TY;
   const unsigned nb = (unsigned)(((h->n1 + mf::RZ_TX - 1) / mf::RZ_TX) * ((h->n2 + ty - 1) / ty) *
                                  ((h->n3 + cz - 1) / cz));
+  mf::ResArgs az = a;
+  az.part = h->part + h->part_stride * h->npend;
   if (zs)
-    hipLaunchKernelGGL(mf::residual_zs_kernel<RES>, dim3(nb), dim3(64, mf::ZS_NW), 0, st, a);
+    hipLaunchKernelGGL(mf::residual_zs_kernel<RES>, dim3(nb), dim3(64, mf::ZS_NW), 0, st, az);
   else
-    hipLaunchKernelGGL(mf::residual_kernel<RES>, dim3(nb), dim3(64, mf::RZ_NW), 0, st, a);
-  hipLaunchKernelGGL(mf::sum_kernel, dim3(1), dim3(1024), 0, st, (const double *)h->part,
-                     (long)nb, norm2_dev);
+    hipLaunchKernelGGL(mf::residual_kernel<RES>, dim3(nb), dim3(64, mf::RZ_NW), 0, st, az);
   HIPCHK(hipGetLastError());
+  h->pend_out[h->npend] = norm2_dev;
+  h->pend_nb = nb;
+  if (++h->npend == mf::NORM_SLOTS) return flush_norms(h, st);
   return MF_OK;
 }
 
@@ -1286,6 +1317,7 @@ void inner_solve(mf_solver3d *h, int precision, double sigma, double *u, bool ac
 int enqueue_check(mf_solver3d *h) {
   auto &p = h->pending;
   int rc = launch_faces(h, p.u, p.bcx, p.bcy, p.bcz, p.st);
+  if (rc == MF_OK) rc = flush_norms(h, p.st);
   if (rc != MF_OK) return rc;
   HIPCHK(hipMemcpyAsync(h->hnorms, h->norms, (p.it + 1) * sizeof(double), hipMemcpyDeviceToHost,
                         p.st));
@@ -1307,6 +1339,7 @@ int mf::solver3d_solve_begin(mf_solver3d *h, const double *g, double sigma, doub
   hipStream_t st = (hipStream_t)stream;
   const long N = (long)h->n1 * h->n2 * h->n3;
   h->pending.active = false;
+  h->npend = 0;
   if (precision == MF_FP64) {
     {
       ProfScope ps(h, P_FOLD, 16.0 * N, st);
