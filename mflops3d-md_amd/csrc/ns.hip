@@ -901,17 +901,49 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
   const double fac1 = s->fac[0];
   double **T = s->T;
   const Extrap ev = extrap_of(s, s->tou, sub), ep = extrap_of(s, s->top - 1, sub);
-  // nonlinear terms of the levels the extrapolation reads
-  for (int m = 0; m < 3; ++m)
-    if (ev.c[m] != 0.0) RC(nonlinear(s, ev.lev[m], st));
+  // The flow-rate force and the velocity boundary values depend on neither
+  // the nonlinear terms nor the LES force: with the side streams they run on
+  // side stream 1 beside the product kernel of the nonlinear terms (the
+  // derivatives those fork to side stream 1 queue after them), and st joins
+  // them before the presolve.
+  hipStream_t sp = st;
+  if (s->side[1]) {
+    sp = s->side[1];
+    NSCHK(hipEventRecord(s->ev_fork, st));
+    NSCHK(hipStreamWaitEvent(sp, s->ev_fork, 0));
+  }
   // constant-flow-rate force from extrapol_v(u), updated at every presolve
   if (s->flow) {
-    hipLaunchKernelGGL(wsum_kernel, dim3(s->nparts), dim3(256), 0, st, (const double *)s->W,
+    hipLaunchKernelGGL(wsum_kernel, dim3(s->nparts), dim3(256), 0, sp, (const double *)s->W,
                        (const double *)s->q[0][ev.lev[0]], (const double *)s->q[0][ev.lev[1]],
                        (const double *)s->q[0][ev.lev[2]], ev.c[0], ev.c[1], ev.c[2], N, s->part,
                        s->d);
-    hipLaunchKernelGGL(flow_kernel, dim3(1), dim3(1024), 0, st, (const double *)s->part, s->nparts,
+    hipLaunchKernelGGL(flow_kernel, dim3(1), dim3(1024), 0, sp, (const double *)s->part, s->nparts,
                        s->vol, fac1, s->fstate);
+  }
+  // velocity boundary values: with the tangential gradient of extrapol_p(phi)
+  // for pt = 1 (add_boundary_gradient), as given otherwise
+  const double *bc[3] = {s->bc0[0], s->bc0[1], s->bc0[2]};
+  if (pt == 1) {
+    BcArgs b;
+    std::memset(&b, 0, sizeof(b));
+    for (int c = 0; c < 3; ++c) { b.bc0[c] = s->bc0[c]; b.bcw[c] = s->bcw[c]; bc[c] = s->bcw[c]; }
+    for (int m = 0; m < 3; ++m) {
+      b.c[m] = ep.c[m];
+      if (ep.c[m] != 0.0 && s->dphi_ok[ep.lev[m]])
+        for (int a = 0; a < 3; ++a) b.dp[m][a] = s->dphi[ep.lev[m]][a];
+    }
+    b.inv_fac1 = 1.0 / fac1;
+    b.d = s->d;
+    hipLaunchKernelGGL(bc_kernel, dim3(nblk(3 * s->nbc)), dim3(256), 0, sp, b);
+  }
+  NSCHK(hipGetLastError());
+  // nonlinear terms of the levels the extrapolation reads
+  for (int m = 0; m < 3; ++m)
+    if (ev.c[m] != 0.0) RC(nonlinear(s, ev.lev[m], st));
+  if (sp != st) {
+    NSCHK(hipEventRecord(s->ev_join[1], sp));
+    NSCHK(hipStreamWaitEvent(st, s->ev_join[1], 0));
   }
   // eddy viscosity and -div(tau) from the latest level (navier3d.f90:68); the
   // level does not change between sub-iterations, so neither does the force
@@ -936,22 +968,6 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
     NSCHK(hipGetLastError());
   }
   const long sx = 2L * (s->ny - 2) * (s->nz - 2), sy = 2L * (s->nx - 2) * (s->nz - 2);
-  // velocity boundary values: with the tangential gradient of extrapol_p(phi)
-  // for pt = 1 (add_boundary_gradient), as given otherwise
-  const double *bc[3] = {s->bc0[0], s->bc0[1], s->bc0[2]};
-  if (pt == 1) {
-    BcArgs b;
-    std::memset(&b, 0, sizeof(b));
-    for (int c = 0; c < 3; ++c) { b.bc0[c] = s->bc0[c]; b.bcw[c] = s->bcw[c]; bc[c] = s->bcw[c]; }
-    for (int m = 0; m < 3; ++m) {
-      b.c[m] = ep.c[m];
-      if (ep.c[m] != 0.0 && s->dphi_ok[ep.lev[m]])
-        for (int a = 0; a < 3; ++a) b.dp[m][a] = s->dphi[ep.lev[m]][a];
-    }
-    b.inv_fac1 = 1.0 / fac1;
-    b.d = s->d;
-    hipLaunchKernelGGL(bc_kernel, dim3(nblk(3 * s->nbc)), dim3(256), 0, st, b);
-  }
   // predicted velocity (pt <= 2), with d_c extrapol_p(p) for pt = 2
   if (pt <= 2) {
     double *dp[3] = {T[21], T[22], T[23]};
