@@ -796,6 +796,14 @@ void launch_gemm(mf_solver3d *h, const TA *A, long lda, long sA, const TB *B, lo
                  long ldc, long sC, int M, int N, int K, int batch, const EpiArgs &ep,
                  hipStream_t st) {
   const int t = gemm_mode_tile(WLEFT ? M : N);
+  // the batched y-transform with a 256-wide N tile and 129..192 rows: one
+  // 192-row M tile and four 64-wide N tiles (the field slice re-read from L2)
+  // give 4 blocks per slice where 64-row M tiles give 3, which at 190 slices
+  // leaves the last round of blocks a quarter full
+  if (!WLEFT && t == 256 && M > 128 && M <= 192 && batch > 1) {
+    launch_tile<T, TA, TB, TC, 192, 64, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
+    return;
+  }
   if (t == 192) {
     if (WLEFT)
       launch_tile<T, TA, TB, TC, 192, 64, EPI>(h, A, lda, sA, B, ldb, C, ldc, sC, M, N, K, batch, ep, st);
