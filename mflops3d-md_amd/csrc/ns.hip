@@ -65,6 +65,19 @@ __global__ void __launch_bounds__(256) prod6_kernel(const double *__restrict__ u
   vv[t] = b * b; vw[t] = b * c; ww[t] = c * c;
 }
 
+// The first presolve's velocity right-hand sides, formed by nl_kernel from
+// the nonlinear term it has just computed (pt = 1, no LES; u, v, w are q^n):
+// rhs_kernel's arithmetic, in its order.  out[0] null: not formed.
+struct NlRhs {
+  double *out[3];
+  const double *n1[3], *n2[3];     // N at the older extrapolation levels
+  const double *q1[3], *q2[3];     // q^n-1, q^n-2
+  const double *force[3];          // or null
+  const double *gflow[3];          // uniform force (device scalar) or null
+  double c0, c1, c2, f2, f3, f4, re;
+  Dims d;
+};
+
 struct NlArgs {
   const double *u, *v, *w;
   const double *g[3][3];   // g[axis][comp] = d q_comp / d x_axis
@@ -72,6 +85,7 @@ struct NlArgs {
   double *out[3];
   long N;
   int nlt;
+  NlRhs r;
 };
 
 // N_c = (u q_x + v q_y + w q_z)/2 + (q u)_x/2 + (q v)_y/2 + (q w)_z/2 (nlt=2),
@@ -80,17 +94,35 @@ __global__ void __launch_bounds__(256) nl_kernel(NlArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.N) return;
   const double u = a.u[t], v = a.v[t], w = a.w[t];
+  const bool rhs = a.r.out[0] != nullptr;
+  const bool edge = rhs && on_edge(t, a.r.d);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double adv = u * a.g[0][c][t] + v * a.g[1][c][t] + w * a.g[2][c][t];
+    double nc;
     if (a.nlt == 1) {
-      a.out[c][t] = adv;
+      nc = adv;
     } else {
-      double r = adv * 0.5;
-      r = r + a.e[0][c][t] * 0.5;
-      r = r + a.e[1][c][t] * 0.5;
-      r = r + a.e[2][c][t] * 0.5;
-      a.out[c][t] = r;
+      nc = adv * 0.5;
+      nc = nc + a.e[0][c][t] * 0.5;
+      nc = nc + a.e[1][c][t] * 0.5;
+      nc = nc + a.e[2][c][t] * 0.5;
+    }
+    a.out[c][t] = nc;
+    if (rhs) {
+      const NlRhs &h = a.r;
+      double ne = h.c0 * nc;
+      if (h.c1 != 0.0) ne += h.c1 * h.n1[c][t];
+      if (h.c2 != 0.0) ne += h.c2 * h.n2[c][t];
+      if (edge) ne = 0.0;
+      double f = h.force[c] ? h.force[c][t] : 0.0;
+      if (h.gflow[c]) f += h.gflow[c][0];
+      double rr = f - ne;
+      if (edge) rr = 0.0;
+      const double q0 = c == 0 ? u : (c == 1 ? v : w);
+      double g = h.f2 * q0 + h.f3 * h.q1[c][t];
+      if (h.f4 != 0.0) g += h.f4 * h.q2[c][t];
+      h.out[c][t] = h.re * (g - rr);
     }
   }
 }
@@ -586,6 +618,9 @@ struct mf_ns {
   mf::ns::Dims d{0, 0, 0};
   std::vector<double> gx, gy, gz;
   int grad_level = -1;  // level whose velocity gradients T[0..8] hold
+  // the first presolve's velocity right-hand sides formed by nl_kernel
+  // (pt = 1, no LES; MF_NS_FUSED_RHS=0: by rhs_kernel)
+  bool fused_rhs = true;
   // LES (navier_LES): model, constant squared, fields -les_f (3) and nu,
   // squared spacings
   int les_type = 0;
@@ -669,7 +704,7 @@ int der3(mf_ns *s, int axis, const double *x0, const double *x1, const double *x
 // velocity gradients and, for the skew-symmetric form, the 9 product
 // derivatives, one batched launch per axis (6 fields: the line recurrences
 // are latency-bound, so a wider batch costs far less than a second launch)
-int nonlinear(mf_ns *s, int L, hipStream_t st) {
+int nonlinear(mf_ns *s, int L, hipStream_t st, const NlRhs *rhs = nullptr) {
   if (s->nl_ok[L]) return MF_OK;
   const long N = s->N;
   double *u = s->q[0][L], *v = s->q[1][L], *w = s->q[2][L];
@@ -698,10 +733,13 @@ int nonlinear(mf_ns *s, int L, hipStream_t st) {
       return der3(s, a, u, v, w, T[3 * a], T[3 * a + 1], T[3 * a + 2], q);
     }));
   }
-  s->grad_level = L;
+  // the fused right-hand sides overwrite the gradients T[0..2] (each point
+  // after its own reads)
+  s->grad_level = rhs ? -1 : L;
   for (int c = 0; c < 3; ++c) n.out[c] = s->nl[c][L];
   n.N = N;
   n.nlt = s->nlt;
+  if (rhs) n.r = *rhs;
   hipLaunchKernelGGL(nl_kernel, dim3(nblk(N)), dim3(256), 0, st, n);
   NSCHK(hipGetLastError());
   s->nl_ok[L] = true;
@@ -805,8 +843,10 @@ void launch_extrap(mf_ns *s, double *const lv[NT], const Extrap &e, double *out,
 // rhs_kernel (rhs1 computed from the cached nonlinear terms ex, or read from
 // rhsb when rhs_in), the pressure gradient dp[c] when given, boundary values
 // bc[c]
+// rhs_done: the right-hand sides are in T[0..2] already (formed by nl_kernel).
 int velocity_solves(mf_ns *s, const Extrap &ev, bool rhs_in, double *const dp[3],
-                    const double *const bc[3], int precision, double tol, hipStream_t st) {
+                    const double *const bc[3], int precision, double tol, hipStream_t st,
+                    bool rhs_done = false) {
   const int *it = s->it;
   const int n1 = it[0], L0 = it[NT - 1], L1 = it[NT - 2], L2 = it[NT - 3];
   const long sx = 2L * (s->ny - 2) * (s->nz - 2), sy = 2L * (s->nx - 2) * (s->nz - 2);
@@ -834,16 +874,18 @@ int velocity_solves(mf_ns *s, const Extrap &ev, bool rhs_in, double *const dp[3]
   };
   if (!s->scu_vw[0]) {
     for (int c = 0; c < 3; ++c) {
-      rhs(c, s->T[0]);
+      double *g = rhs_done ? s->T[c] : s->T[0];
+      if (!rhs_done) rhs(c, g);
       NSCHK(hipGetLastError());
-      RC(mf_solver3d_solve(s->scu, s->T[0], sigma, s->q[c][n1], bc[c], bc[c] + sx,
+      RC(mf_solver3d_solve(s->scu, g, sigma, s->q[c][n1], bc[c], bc[c] + sx,
                            bc[c] + sx + sy, precision, tol, 0, nullptr, st));
     }
   } else {
     // right-hand sides into T[0..2] on st; u's solve on st, v's and w's on
     // the side streams after them; the host waits for all three (mixed
     // precision checks each residual), and st waits for the side streams
-    for (int c = 0; c < 3; ++c) rhs(c, s->T[c]);
+    if (!rhs_done)
+      for (int c = 0; c < 3; ++c) rhs(c, s->T[c]);
     NSCHK(hipGetLastError());
     NSCHK(hipEventRecord(s->ev_fork, st));
     for (int k = 0; k < 2; ++k) NSCHK(hipStreamWaitEvent(s->side[k], s->ev_fork, 0));
@@ -938,9 +980,33 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
     hipLaunchKernelGGL(bc_kernel, dim3(nblk(3 * s->nbc)), dim3(256), 0, sp, b);
   }
   NSCHK(hipGetLastError());
-  // nonlinear terms of the levels the extrapolation reads
-  for (int m = 0; m < 3; ++m)
+  // nonlinear terms of the levels the extrapolation reads, the older ones
+  // first: for pt = 1 without LES the newest one's kernel also forms the
+  // velocity right-hand sides (into T[0..2]), which read the older ones
+  const bool fuse_rhs = s->fused_rhs && pt == 1 && sub == 1 && !s->les_type && !s->sgs &&
+                        ev.c[0] != 0.0 && ev.lev[0] == L0 && !s->nl_ok[ev.lev[0]];
+  for (int m = 2; m >= 1; --m)
     if (ev.c[m] != 0.0) RC(nonlinear(s, ev.lev[m], st));
+  if (ev.c[0] != 0.0) {
+    NlRhs h;
+    std::memset(&h, 0, sizeof(h));
+    if (fuse_rhs) {
+      for (int c = 0; c < 3; ++c) {
+        h.out[c] = T[c];
+        h.n1[c] = s->nl[c][ev.lev[1]];
+        h.n2[c] = s->nl[c][ev.lev[2]];
+        h.q1[c] = s->q[c][L1];
+        h.q2[c] = s->q[c][L2];
+        h.force[c] = s->has_force[c] ? s->force[c] : nullptr;
+        h.gflow[c] = (c == 0 && s->flow) ? s->fstate : nullptr;
+      }
+      h.c0 = ev.c[0]; h.c1 = ev.c[1]; h.c2 = ev.c[2];
+      h.f2 = s->fac[1]; h.f3 = s->fac[2]; h.f4 = s->tou == 3 ? s->fac[3] : 0.0;
+      h.re = s->re;
+      h.d = s->d;
+    }
+    RC(nonlinear(s, ev.lev[0], st, fuse_rhs ? &h : nullptr));
+  }
   if (sp != st) {
     NSCHK(hipEventRecord(s->ev_join[1], sp));
     NSCHK(hipStreamWaitEvent(st, s->ev_join[1], 0));
@@ -975,7 +1041,7 @@ int substep(mf_ns *s, int sub, int precision, double tol, bool *done, hipStream_
       launch_extrap(s, s->p, ep, T[20], st);
       RC(fork3(s, st, [&](int a, hipStream_t q) { return der(s, a, T[20], dp[a], q); }));
     }
-    RC(velocity_solves(s, ev, false, pt == 2 ? dp : nullptr, bc, precision, tol, st));
+    RC(velocity_solves(s, ev, false, pt == 2 ? dp : nullptr, bc, precision, tol, st, fuse_rhs));
   }
   // pressure: rhs_p, fphi = div rhs_p, Neumann values, solve
   {
@@ -1137,6 +1203,7 @@ extern "C" int mf_ns_create(const double *gx, int nx, const double *gy, int ny, 
   if (rc == MF_OK) rc = mf_solver3d_create(gx, nx, gy, ny, gz, nz, bp, so, &s->scp);
   bool conc = true;
   if (const char *env = std::getenv("MF_NS_CONCURRENT")) conc = std::atoi(env) != 0;
+  if (const char *env = std::getenv("MF_NS_FUSED_RHS")) s->fused_rhs = std::atoi(env) != 0;
   for (int k = 0; conc && k < 2 && rc == MF_OK; ++k)
     rc = mf_solver3d_create(gx, nx, gy, ny, gz, nz, bu, so, &s->scu_vw[k]);
   if (conc && rc == MF_OK) {

@@ -402,6 +402,29 @@ def test_ns_concurrent_velocity_solves_match_sequential(monkeypatch, prec):
         assert np.array_equal(a, b), (name, np.abs(a - b).max())
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("tou", [1, 2, 3])
+def test_ns_fused_right_hand_sides_match_separate(monkeypatch, tou):
+    """The first presolve's velocity right-hand sides formed inside the
+    nonlinear-term kernel (pt = 1, the default) against rhs_kernel
+    (MF_NS_FUSED_RHS=0): the same arithmetic in the same order, so the
+    fields agree bit for bit."""
+    import mflops as mf
+    mf.load()
+    gx, gy, gz, u, v, w, bc = _channel((40, 48, 36))
+    re, dt, steps = 3000.0, 1e-3, 4
+    runs = {}
+    for fused in ("1", "0"):
+        print("run ns fused_rhs=%s tou=%d" % (fused, tou), flush=True)
+        monkeypatch.setenv("MF_NS_FUSED_RHS", fused)
+        dev = _gpu_ns(mf, gx, gy, gz, u, v, w, bc, re, dt, tou=tou, top=tou)
+        dev.set_flow_rate(True, 2.0 / re)
+        dev.step(steps, mf.MF_FP64)
+        runs[fused] = [mf.to_host(t) for t in dev.get()]
+    for name, a, b in zip(("u", "v", "w", "p", "phi"), runs["1"], runs["0"]):
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+
+
 def _orc_fields(orc):
     return list(orc.velocity()) + [orc.pressure(), orc.phi_latest()]
 
